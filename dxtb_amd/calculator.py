@@ -1,0 +1,287 @@
+"""Drop-in ``GFN1Calculator`` on PyTorch-ROCm tensors.
+
+Mirrors upstream's calculator surface for the GFN1 single point
+(src/dxtb/_src/calculators/gfn1.py:38-70, types/base.py:451-734,
+types/abc.py:88-170,429-486, types/energy.py:78-441, types/autograd.py:80-201,
+types/analytical.py:61-222): same constructor arguments, padded-batch
+convention (atomic number 0 = padding), ``get_energy`` / ``get_forces`` /
+``get_charges`` / ``reset``, the same option names, exception and warning
+types.  All arithmetic runs in the HIP library (FP64); the Python code only
+lays out the batch and moves tensors.
+
+Differences from upstream, by design:
+* dtype must be float64 (the kernels are FP64 only); ``dtype=None`` means
+  float64.
+* D3 dispersion needs the reference C6 tables of tad-dftd3, which are not
+  part of this package; the dispersion term is left out (with a warning
+  unless "disp" is in ``opts["exclude"]``).
+* The SCF always mixes the potential with the Anderson mixer (upstream's
+  default full mode falls back to Anderson as well,
+  src/dxtb/_src/scf/unrolling/base.py:76-99); ``scf_mode`` and ``mixer``
+  are accepted for compatibility and change neither the fixed point nor the
+  result.
+"""
+
+from __future__ import annotations
+
+import warnings
+from typing import Any
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import constants as C
+from .engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
+from .exceptions import (
+    DeviceError,
+    DtypeError,
+    PropertyNotImplementedError,
+    SCFConvergenceError,
+    SCFConvergenceWarning,
+)
+from .param import SpeciesTables
+
+_DISP_WARNED = False
+
+
+def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
+    o = ScfOptions()
+    rest = dict(opts)
+    for key, attr in (
+        ("maxiter", "maxiter"), ("mix_guess", "mix_guess"), ("damp", "damp"),
+        ("damp_init", "damp_init"), ("damp_generations", "generations"),
+        ("damp_diagonal_offset", "diag_offset"), ("x_atol", "x_atol"),
+        ("x_atol_max", "x_atol_max"), ("fermi_etemp", "fermi_etemp"),
+        ("fermi_maxiter", "fermi_maxiter"), ("fermi_thresh", "fermi_thresh"),
+    ):
+        if key in rest:
+            setattr(o, attr, rest.pop(key))
+    scp = str(rest.pop("scp_mode", "potential")).lower()
+    if scp not in ("potential", "1"):
+        raise NotImplementedError(f"scp_mode '{scp}': only 'potential' is implemented")
+    if rest.pop("damp_soft_start", True) is not True:
+        raise NotImplementedError("damp_soft_start=False is not implemented")
+    if rest.pop("damp_dynamic", False):
+        raise NotImplementedError("damp_dynamic is not implemented")
+    if not 1 <= int(o.generations) <= 5:
+        raise ValueError("damp_generations must be in 1..5")
+    return o, rest
+
+
+class _EnergyFn(torch.autograd.Function):
+    """Energy with a hand-written backward: the analytic gradient kernel."""
+
+    @staticmethod
+    def forward(ctx, positions, calc, chrg, spin):
+        res = calc._run(positions.detach(), chrg, spin, forces=True)
+        ctx.save_for_backward(res["gradient_full"])
+        return res["energy"].clone()
+
+    @staticmethod
+    def backward(ctx, grad_out):
+        (g,) = ctx.saved_tensors
+        if g.ndim == 2:
+            return grad_out * g, None, None, None
+        return grad_out[:, None, None] * g, None, None, None
+
+
+class GFN1Calculator:
+    """GFN1-xTB calculator (drop-in for upstream ``dxtb.GFN1Calculator``)."""
+
+    def __init__(self, numbers: torch.Tensor, par: Any = None, *, classical=None,
+                 interaction=None, opts: dict | None = None, timer: bool = False,
+                 device: torch.device | str | None = None, dtype: torch.dtype | None = None,
+                 **kwargs: Any) -> None:
+        if not isinstance(numbers, torch.Tensor):
+            numbers = torch.as_tensor(numbers)
+        if numbers.dtype not in (torch.int16, torch.int32, torch.int64):
+            raise DtypeError(f"Dtype '{numbers.dtype}' of 'numbers' must be an integer type.")
+        if classical is not None or interaction is not None:
+            raise NotImplementedError("custom classical/interaction components are not supported")
+        self.dtype = torch.float64 if dtype is None else dtype
+        if self.dtype != torch.float64:
+            raise DtypeError("this calculator computes in float64 only")
+        self.device = torch.device(device) if device is not None else numbers.device
+        self.numbers = numbers.to(self.device)
+        self.batched = numbers.ndim == 2
+        opts = dict(opts or {})
+        opts.pop("verbosity", None)
+        self.force_convergence = bool(opts.pop("force_convergence", False))
+        exclude = opts.pop("exclude", [])
+        self.exclude = {exclude} if isinstance(exclude, str) else set(exclude)
+        for k in ("scf_mode", "mixer", "int_driver", "int_level", "int_uplo", "batch_mode",
+                  "method", "strict", "guess", "f_atol", "fermi_partition", "cache_enabled",
+                  "max_element", "anomaly", "grad", "skip_compat_checks", "int_cutoff"):
+            opts.pop(k, None)
+        self.scf_opts, rest = _opts_to_scf(opts)
+        self.scf_opts.jacobi_max_sweeps = int(kwargs.pop("jacobi_max_sweeps", 40))
+        self.layout_padded = bool(kwargs.pop("padded", True))
+        if rest:
+            raise TypeError(f"unknown options: {sorted(rest)}")
+        nums = self.numbers.detach().cpu().numpy().astype(np.int64)
+        self._mols = [r[r > 0] for r in nums] if self.batched else [nums[nums > 0]]
+        self._mask = (self.numbers > 0)
+        zs = np.unique(np.concatenate(self._mols))
+        for z in zs:
+            if int(z) not in C.SUPPORTED_ELEMENTS:
+                raise NotImplementedError(
+                    f"Element Z={int(z)} not supported (supported: {C.SUPPORTED_ELEMENTS}).")
+        self.tables = SpeciesTables(zs)
+        self._dtables = None
+        self._pipe = None
+        self._pipe_key = None
+        self._cache: dict = {}
+        global _DISP_WARNED
+        if not ({"disp", "all", "scf"} & self.exclude) and not _DISP_WARNED:
+            warnings.warn("D3 dispersion is not available in this build (its reference C6 data "
+                          "ship with tad-dftd3); energies and forces exclude it. Pass "
+                          "opts={'exclude': ['disp']} to silence this warning.", UserWarning)
+            _DISP_WARNED = True
+
+    # ------------------------------------------------------------------ core
+    def reset(self) -> None:
+        self._cache = {}
+
+    def _check(self, positions: torch.Tensor) -> None:
+        if positions.dtype != self.dtype:
+            raise DtypeError(f"positions dtype {positions.dtype} != calculator dtype {self.dtype}")
+        if positions.device != self.device:
+            raise DeviceError(f"positions on {positions.device}, calculator on {self.device}")
+        if positions.shape[:-1] != self.numbers.shape or positions.shape[-1] != 3:
+            raise ValueError("positions must have shape (*numbers.shape, 3)")
+
+    def _charges_array(self, chrg) -> np.ndarray:
+        B = len(self._mols)
+        if isinstance(chrg, torch.Tensor):
+            c = chrg.detach().cpu().numpy().astype(np.float64).reshape(-1)
+        else:
+            c = np.asarray([float(chrg)])
+        if self.batched and c.size == 1:
+            if c[0] != 0.0:
+                raise ValueError("Cannot set charge for batched calculations with a single "
+                                 "float or integer. Please provide a 1D tensor of charges.")
+            c = np.zeros(B)
+        if c.size != B:
+            raise ValueError("charge tensor does not match the batch")
+        return c
+
+    def _pipeline(self, chrg, spin) -> Pipeline:
+        c = self._charges_array(chrg)
+        sp = None if spin is None else np.asarray(
+            spin.detach().cpu().numpy() if isinstance(spin, torch.Tensor) else spin, dtype=np.float64)
+        key = (c.tobytes(), None if sp is None else sp.tobytes())
+        if self._pipe is None or self._pipe_key != key:
+            if self._dtables is None:
+                self._dtables = DeviceTables(self.tables, self.device)
+            layout = BatchLayout(self._mols, self.tables, padded=self.batched and self.layout_padded,
+                                 chrg=c, spin=sp)
+            self._pipe = Pipeline(layout, self._dtables, self.device)
+            self._pipe_key = key
+        return self._pipe
+
+    def _run(self, positions: torch.Tensor, chrg=0, spin=None, forces: bool = False) -> dict:
+        self._check(positions)
+        ckey = (positions.data_ptr(), positions._version, id(positions), forces,
+                self._charges_array(chrg).tobytes(), str(spin))
+        hit = self._cache.get("run")
+        if hit is not None and hit[0] == ckey:
+            return hit[1]
+        if hit is not None and forces is False and hit[0][:3] == ckey[:3] and hit[0][4:] == ckey[4:]:
+            return hit[1]
+        pipe = self._pipeline(chrg, spin)
+        pos = positions.detach()
+        flat = pos[self._mask].contiguous() if self.batched else pos[self._mask].contiguous()
+        pipe.run(flat, self.scf_opts, forces=forces)
+        rec = pipe.records().clone()
+        status = rec[:, _lib.OUT_STATUS].to(torch.int64).cpu().numpy()
+        self._raise_on_status(status)
+        exclude_rep = bool({"rep"} & self.exclude)
+        exclude_scf = bool({"scf", "all"} & self.exclude)
+        e = torch.zeros(len(self._mols), dtype=torch.float64, device=self.device)
+        if not exclude_scf:
+            e = e + rec[:, _lib.OUT_ESCF] + rec[:, _lib.OUT_FENERGY]
+        if not exclude_rep and "all" not in self.exclude:
+            e = e + rec[:, _lib.OUT_EREP]
+        if exclude_rep or exclude_scf:
+            if forces:
+                raise NotImplementedError("forces with excluded components are not implemented")
+        res = {"energy": e if self.batched else e[0], "records": rec, "pipeline": pipe}
+        if forces:
+            g = torch.zeros(positions.shape, dtype=torch.float64, device=self.device)
+            g[self._mask] = pipe.gradient().clone()
+            res["gradient_full"] = g
+        self._cache["run"] = (ckey, res)
+        return res
+
+    def _raise_on_status(self, status: np.ndarray) -> None:
+        if np.any(status & _lib.MOL_FERMI):
+            raise RuntimeError("Fermi energy failed to converge.")
+        if np.any(status & _lib.MOL_NONFINITE):
+            raise RuntimeError("Non-finite values met in the SCF.")
+        if np.any(status & _lib.MOL_MAXITER):
+            msg = (f"SCF does not converge after {self.scf_opts.maxiter} cycles using Anderson "
+                   f"mixing with a damping factor of {self.scf_opts.damp}.")
+            if self.force_convergence:
+                raise SCFConvergenceError(msg)
+            warnings.warn(msg, SCFConvergenceWarning)
+
+    # ----------------------------------------------------------- properties
+    def singlepoint(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> dict:
+        res = self._run(positions, chrg, spin, forces=False)
+        rec = res["records"]
+        out = {
+            "energy": res["energy"],
+            "scf": rec[:, _lib.OUT_ESCF], "fenergy": rec[:, _lib.OUT_FENERGY],
+            "repulsion": rec[:, _lib.OUT_EREP],
+            "iterations": rec[:, _lib.OUT_ITER].to(torch.int64),
+        }
+        if not self.batched:
+            out = {k: (v[0] if k != "energy" else v) for k, v in out.items()}
+        return out
+
+    def energy(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> torch.Tensor:
+        if positions.requires_grad and torch.is_grad_enabled():
+            return _EnergyFn.apply(positions, self, chrg, spin)
+        return self._run(positions, chrg, spin, forces=False)["energy"]
+
+    get_energy = energy
+
+    def forces(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> torch.Tensor:
+        if not positions.requires_grad:
+            raise RuntimeError("Position tensor needs ``requires_grad=True``.")
+        e = self.energy(positions, chrg, spin)
+        (g,) = torch.autograd.grad(e.sum(), positions)
+        return -g
+
+    def get_forces(self, positions: torch.Tensor, chrg=0, spin=None, grad_mode: str = "autograd",
+                   **kwargs) -> torch.Tensor:
+        if grad_mode not in ("autograd", "backward", "functorch", "row", "analytical"):
+            raise PropertyNotImplementedError(f"grad_mode '{grad_mode}'")
+        if grad_mode == "analytical":
+            return self.forces_analytical(positions, chrg, spin)
+        return self.forces(positions, chrg, spin)
+
+    def forces_analytical(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> torch.Tensor:
+        return -self._run(positions, chrg, spin, forces=True)["gradient_full"]
+
+    def get_charges(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> torch.Tensor:
+        """Orbital-resolved Mulliken charges, shape (..., nao) (types/abc.py:429-486)."""
+        res = self._run(positions, chrg, spin, forces=False)
+        pipe = res["pipeline"]
+        L = pipe.L
+        nmax = int(L.nao.max())
+        out = torch.zeros((L.B, nmax), dtype=torch.float64, device=self.device)
+        for b in range(L.B):
+            out[b, : int(L.nao[b])] = pipe.orbital_charges(b)
+        return out if self.batched else out[0]
+
+    def get_iterations(self, positions, chrg=0, spin=None):
+        return self.singlepoint(positions, chrg, spin)["iterations"]
+
+    def get_mo_energies(self, positions, chrg=0, spin=None):
+        res = self._run(positions, chrg, spin, forces=False)
+        pipe = res["pipeline"]
+        L = pipe.L
+        out = [pipe.emo[int(L.vec0[b]): int(L.vec0[b]) + int(L.n[b])].clone() for b in range(L.B)]
+        return torch.stack(out) if self.batched and L.padded else (out if self.batched else out[0])

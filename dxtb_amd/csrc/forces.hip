@@ -1,0 +1,191 @@
+// Analytic nuclear gradient of the GFN1 energy (gfx950, FP64).
+//
+// Because the SCF energy is variational, dE/dR needs no SCF response: it is
+// the contraction of the converged density P, the energy-weighted density W
+// and the orbital potential v with derivatives of the integrals, which are
+// recomputed here per atom pair instead of being stored
+// (upstream stores dS/dR as an (n, n, 3) tensor,
+//  src/dxtb/_src/integral/driver/pytorch/impls/overlap.py:246-349).
+//
+//   dE/dR_A = sum_{mu in A, nu not in A} [2 P H~ - 2 W - P (v_mu + v_nu)] dS/dR_A
+//           + sum_B 2 (P H~ S)_AB (dPi/dR)/(R Pi) (R_A - R_B)
+//           + sum_C dE/dCN_C dCN_C/dR_A
+//           - sum_B q_a q_b gamma_ab^3 (R_A - R_B)            (ES2, fixed q)
+//           + repulsion
+// (src/dxtb/_src/xtb/gfn1.py:185-408, src/dxtb/_src/calculators/types/
+//  analytical.py:61-222, coulomb/secondorder.py:873-926,
+//  classicals/repulsion/base.py:337-406, src/dxtb/_src/ncoord/utils.py:30-52).
+#include "common.h"
+
+namespace xtb {
+
+__global__ void k_zero(double* __restrict__ a, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = 0.0;
+}
+
+#define XTB_MAX_US_LDS_F 64
+
+__global__ void __launch_bounds__(256) k_grad_pairs(xtb_batch b, xtb_params par,
+                                                    const double* __restrict__ pos,
+                                                    const double* __restrict__ cn,
+                                                    const double* __restrict__ gam,
+                                                    xtb_scf_state st, double* __restrict__ dedcn,
+                                                    double* __restrict__ grad) {
+    __shared__ double s_alpha[XTB_MAX_US_LDS_F * XTB_MAXPRIM];
+    __shared__ double s_coeff[XTB_MAX_US_LDS_F * XTB_MAXPRIM];
+    for (int i = threadIdx.x; i < par.nus * XTB_MAXPRIM; i += blockDim.x) {
+        s_alpha[i] = par.us_alpha[i];
+        s_coeff[i] = par.us_coeff[i];
+    }
+    __syncthreads();
+    const int* tl = b.tile_at + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int A = tl[1] + threadIdx.x / 16;
+    const int B = tl[2] + threadIdx.x % 16;
+    const int nat = b.mol_nat[m];
+    if (A >= nat || B >= nat || B < A) return;
+    const int n = b.mol_n[m];
+    const int at0 = b.mol_atom0[m], sh0 = b.mol_sh0[m], vec0 = b.mol_vec0[m];
+    const size_t mat0 = (size_t)b.mol_mat0[m];
+    const double* P = st.P + mat0;
+    const double* W = st.T + mat0;
+    const double* v = st.pot + vec0;
+    const int ga = at0 + A, gb = at0 + B;
+    const int sa0 = b.atom_sh0[ga], nsa = b.atom_nsh[ga];
+    if (A == B) {
+        // on-site part of dE/dCN: sum_{i in A} (-kcn_i) sum_{mu in i} P_mumu
+        double d = 0.0;
+        for (int i = sa0; i < sa0 + nsa; ++i) {
+            const int us = b.sh_us[sh0 + i];
+            const int o0 = b.sh_ao[sh0 + i], nl = 2 * par.us_l[us] + 1;
+            double pp = 0.0;
+            for (int x = 0; x < nl; ++x) pp += P[(size_t)(o0 + x) * n + (o0 + x)];
+            d -= par.us_kcn[us] * pp;
+        }
+        atomicAdd(dedcn + ga, d);
+        return;
+    }
+    const int sb0 = b.atom_sh0[gb], nsb = b.atom_nsh[gb];
+    const double R[3] = {pos[3 * ga] - pos[3 * gb], pos[3 * ga + 1] - pos[3 * gb + 1],
+                         pos[3 * ga + 2] - pos[3 * gb + 2]};
+    const double dist = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    const int spa = b.atom_sp[ga], spb = b.atom_sp[gb];
+    const double rab = par.sp_rad[spa] + par.sp_rad[spb];
+    const int nsh = b.mol_nsh[m];
+    const double* g = gam + (size_t)b.mol_gam0[m];
+    const double* qsh = st.qsh + sh0;
+    double gvec[3] = {0.0, 0.0, 0.0};
+    double radial = 0.0;  // coefficient of (R_A - R_B)
+    double dA = 0.0, dB = 0.0;
+    const bool in_cut = dist < par.int_cutoff && dist > 0.1;
+    for (int i = sa0; i < sa0 + nsa; ++i) {
+        const int usi = b.sh_us[sh0 + i];
+        const int li = par.us_l[usi], ni = 2 * li + 1, oi = b.sh_ao[sh0 + i];
+        const double hi = par.us_level[usi] - par.us_kcn[usi] * cn[ga];
+        for (int j = sb0; j < sb0 + nsb; ++j) {
+            const int usj = b.sh_us[sh0 + j];
+            const int lj = par.us_l[usj], nj = 2 * lj + 1, oj = b.sh_ao[sh0 + j];
+            // ES2 with fixed charges: d gamma / dR_A = -gamma^3 (R_A - R_B)
+            const double gij = g[(size_t)i * nsh + j];
+            radial -= gij * gij * gij * qsh[i] * qsh[j];
+            if (!in_cut) continue;
+            const double hj = par.us_level[usj] - par.us_kcn[usj] * cn[gb];
+            double Sb[3][3], dSb[3][3][3];
+            shell_overlap<true>(li, lj, par.us_nprim[usi], s_alpha + usi * XTB_MAXPRIM,
+                                s_coeff + usi * XTB_MAXPRIM, par.us_nprim[usj],
+                                s_alpha + usj * XTB_MAXPRIM, s_coeff + usj * XTB_MAXPRIM, R, Sb,
+                                dSb);
+            double pi, dpi;
+            dist_poly(dist, rab, par.us_shpoly[usi], par.us_shpoly[usj], pi, dpi);
+            const double kk = par.us_K[usi * par.nus + usj];
+            const double ht = 0.5 * (hi + hj) * kk * pi;
+            double phs = 0.0, ps = 0.0;
+            for (int x = 0; x < ni; ++x) {
+                const double vm = v[oi + x];
+                for (int y = 0; y < nj; ++y) {
+                    const size_t e = (size_t)(oi + x) * n + (oj + y);
+                    const double pmn = P[e], wmn = W[e];
+                    const double sval = 2.0 * (pmn * ht - wmn) - pmn * (vm + v[oj + y]);
+                    gvec[0] += sval * dSb[0][x][y];
+                    gvec[1] += sval * dSb[1][x][y];
+                    gvec[2] += sval * dSb[2][x][y];
+                    phs += pmn * Sb[x][y];
+                    ps += pmn * Sb[x][y];
+                }
+            }
+            radial += 2.0 * phs * ht * dpi / pi;
+            const double fac = kk * pi;
+            dA -= ps * fac * par.us_kcn[usi];
+            dB -= ps * fac * par.us_kcn[usj];
+        }
+    }
+    // repulsion (repulsion/base.py:337-406)
+    if (dist <= par.rep_cutoff) {
+        const double alpha = sqrt(par.sp_arep[spa] * par.sp_arep[spb] + 2.2250738585072014e-308);
+        const double rk = pow(dist, par.kexp);
+        const double e = par.sp_zeff[spa] * par.sp_zeff[spb] * exp(-alpha * rk) / dist;
+        const double dedr = -e * (1.0 / dist + alpha * par.kexp * rk / dist);
+        radial += dedr / dist;
+    }
+    for (int k = 0; k < 3; ++k) {
+        const double gk = gvec[k] + radial * R[k];
+        atomicAdd(grad + 3 * ga + k, gk);
+        atomicAdd(grad + 3 * gb + k, -gk);
+    }
+    atomicAdd(dedcn + ga, dA);
+    atomicAdd(dedcn + gb, dB);
+}
+
+// CN chain rule: dE/dR += sum_A dE/dCN_A dCN_A/dR  (ncoord/utils.py:30-52)
+__global__ void __launch_bounds__(256) k_grad_cn(xtb_batch b, xtb_params par,
+                                                 const double* __restrict__ pos,
+                                                 const double* __restrict__ dedcn,
+                                                 double* __restrict__ grad) {
+    const int* tl = b.tile_at + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int A = tl[1] + threadIdx.x / 16;
+    const int B = tl[2] + threadIdx.x % 16;
+    const int nat = b.mol_nat[m];
+    if (A >= nat || B >= nat || B <= A) return;
+    const int at0 = b.mol_atom0[m];
+    const int ga = at0 + A, gb = at0 + B;
+    const double R[3] = {pos[3 * ga] - pos[3 * gb], pos[3 * ga + 1] - pos[3 * gb + 1],
+                         pos[3 * ga + 2] - pos[3 * gb + 2]};
+    const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+    if (r > par.cn_cutoff) return;
+    const double rc = par.sp_rcov[b.atom_sp[ga]] + par.sp_rcov[b.atom_sp[gb]];
+    const double e = exp(-par.cn_steep * (rc / r - 1.0));
+    const double dcf = -par.cn_steep * rc * e / (r * r * (1.0 + e) * (1.0 + e));
+    const double f = (dedcn[ga] + dedcn[gb]) * dcf / r;
+    for (int k = 0; k < 3; ++k) {
+        atomicAdd(grad + 3 * ga + k, f * R[k]);
+        atomicAdd(grad + 3 * gb + k, -f * R[k]);
+    }
+}
+
+}  // namespace xtb
+
+using namespace xtb;
+
+extern "C" int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
+                              const double* positions, const double* cn, const double* gamma,
+                              const xtb_scf_state* state, double* dedcn, double* grad,
+                              void* stream) {
+    if (!batch || !par || !positions || !cn || !gamma || !state || !dedcn || !grad)
+        return XTB_EINVAL;
+    if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
+    if (!state->P || !state->T || !state->pot || !state->qsh) return XTB_EINVAL;
+    if (par->nus > XTB_MAX_US_LDS_F) return XTB_ETOOBIG;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, s, grad, (int64_t)3 * batch->nat_total);
+    hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, s, dedcn, (int64_t)batch->nat_total);
+    if (batch->ntile_at > 0) {
+        hipLaunchKernelGGL(k_grad_pairs, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
+                           positions, cn, gamma, *state, dedcn, grad);
+        hipLaunchKernelGGL(k_grad_cn, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
+                           positions, dedcn, grad);
+    }
+    return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP;
+}

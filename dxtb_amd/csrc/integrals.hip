@@ -1,0 +1,271 @@
+// Overlap, core Hamiltonian, coordination number, ES2 Coulomb matrix and
+// classical repulsion for a ragged batch of molecules (gfx950, FP64).
+//
+// Work decomposition: the shell-pair (atom-pair) space of every molecule is
+// cut into 16x16 tiles of its upper triangle; one 256-thread workgroup per
+// tile, one thread per shell (atom) pair.  The tile list is built on the
+// host once per batch.  Primitive exponents and contraction coefficients of
+// all unique shells are staged in LDS at workgroup start.
+#include "common.h"
+
+namespace xtb {
+
+#define XTB_MAX_US_LDS 64
+
+// ---------------------------------------------------------------------------
+// coordination number: D3 counting with exp_count,
+// CN_A = sum_{B != A, R <= cutoff} 1 / (1 + exp(-k (Rcov_AB / R - 1)))
+// (tad_mctc cn_d3 as called at src/dxtb/_src/xtb/gfn1.py:57-64)
+// ---------------------------------------------------------------------------
+__global__ void k_cn(xtb_batch b, xtb_params par, const double* __restrict__ pos,
+                     double* __restrict__ cn) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= b.nat_total) return;
+    const int m = b.atom_mol[a];
+    const int a0 = b.mol_atom0[m], na = b.mol_nat[m];
+    const double ra = par.sp_rcov[b.atom_sp[a]];
+    const double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
+    double s = 0.0;
+    for (int j = a0; j < a0 + na; ++j) {
+        if (j == a) continue;
+        const double dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (r > par.cn_cutoff) continue;
+        const double rc = ra + par.sp_rcov[b.atom_sp[j]];
+        s += 1.0 / (1.0 + exp(-par.cn_steep * (rc / r - 1.0)));
+    }
+    cn[a] = s;
+}
+
+__device__ inline void stage_prims(const xtb_params& par, double* s_alpha, double* s_coeff) {
+    const int nn = par.nus * XTB_MAXPRIM;
+    for (int i = threadIdx.x; i < nn; i += blockDim.x) {
+        s_alpha[i] = par.us_alpha[i];
+        s_coeff[i] = par.us_coeff[i];
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// S and H0 (src/dxtb/_src/integral/driver/pytorch/impls/overlap.py:147-243,
+// src/dxtb/_src/xtb/base.py:251-360).  Off-site blocks:
+//   S_ij from the contracted s/p overlap (cutoff 50 bohr, same-centre mask),
+//   H0_ij = S_ij * 1/2 (h_i + h_j) * K_ij * Pi_ij,  h = level - kcn * CN.
+// On-site blocks: S = identity on the shell's own orbitals, H0 = h on the
+// diagonal, zero between different shells of the same atom.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
+                                              const double* __restrict__ pos,
+                                              const double* __restrict__ cn,
+                                              double* __restrict__ S, double* __restrict__ H0) {
+    __shared__ double s_alpha[XTB_MAX_US_LDS * XTB_MAXPRIM];
+    __shared__ double s_coeff[XTB_MAX_US_LDS * XTB_MAXPRIM];
+    stage_prims(par, s_alpha, s_coeff);
+    const int* tl = b.tile_sh + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int i = tl[1] + threadIdx.x / 16;
+    const int j = tl[2] + threadIdx.x % 16;
+    const int nsh = b.mol_nsh[m];
+    if (i >= nsh || j >= nsh || j < i) return;
+    const int n = b.mol_n[m];
+    const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
+    const size_t mat0 = (size_t)b.mol_mat0[m];
+    const int usi = b.sh_us[sh0 + i], usj = b.sh_us[sh0 + j];
+    const int ati = b.sh_atom[sh0 + i], atj = b.sh_atom[sh0 + j];
+    const int li = par.us_l[usi], lj = par.us_l[usj];
+    const int ni = 2 * li + 1, nj = 2 * lj + 1;
+    const int oi = b.sh_ao[sh0 + i], oj = b.sh_ao[sh0 + j];
+    const double hi = par.us_level[usi] - par.us_kcn[usi] * cn[at0 + ati];
+    const double hj = par.us_level[usj] - par.us_kcn[usj] * cn[at0 + atj];
+    double blk[3][3], hblk[3][3];
+    for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) { blk[x][y] = 0.0; hblk[x][y] = 0.0; }
+    if (ati == atj) {
+        if (i == j)
+            for (int x = 0; x < ni; ++x) { blk[x][x] = 1.0; hblk[x][x] = hi; }
+    } else {
+        const int ga = at0 + ati, gb = at0 + atj;
+        double R[3] = {pos[3 * ga] - pos[3 * gb], pos[3 * ga + 1] - pos[3 * gb + 1],
+                       pos[3 * ga + 2] - pos[3 * gb + 2]};
+        const double dist = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        if (dist < par.int_cutoff && dist > 0.1) {
+            double dummy[3][3][3];
+            shell_overlap<false>(li, lj, par.us_nprim[usi], s_alpha + usi * XTB_MAXPRIM,
+                                 s_coeff + usi * XTB_MAXPRIM, par.us_nprim[usj],
+                                 s_alpha + usj * XTB_MAXPRIM, s_coeff + usj * XTB_MAXPRIM, R,
+                                 blk, dummy);
+            const double rab = par.sp_rad[b.atom_sp[ga]] + par.sp_rad[b.atom_sp[gb]];
+            double pi, dpi;
+            dist_poly(dist, rab, par.us_shpoly[usi], par.us_shpoly[usj], pi, dpi);
+            const double fac = 0.5 * (hi + hj) * par.us_K[usi * par.nus + usj] * pi;
+            for (int x = 0; x < ni; ++x)
+                for (int y = 0; y < nj; ++y) hblk[x][y] = blk[x][y] * fac;
+        }
+    }
+    for (int x = 0; x < ni; ++x)
+        for (int y = 0; y < nj; ++y) {
+            const size_t e1 = mat0 + (size_t)(oi + x) * n + (oj + y);
+            const size_t e2 = mat0 + (size_t)(oj + y) * n + (oi + x);
+            S[e1] = blk[x][y];
+            H0[e1] = hblk[x][y];
+            S[e2] = blk[x][y];
+            H0[e2] = hblk[x][y];
+        }
+}
+
+// Padding orbitals of the padded layout: unit diagonal in S (as the
+// eigensolver sees it upstream, scf/unrolling/base.py:122-139), zero H0.
+__global__ void k_pad_fill(xtb_batch b, double* __restrict__ S, double* __restrict__ H0) {
+    const int m = blockIdx.y;
+    const int n = b.mol_n[m], nao = b.mol_nao[m];
+    if (n == nao) return;
+    const size_t mat0 = (size_t)b.mol_mat0[m];
+    const size_t tot = (size_t)n * n;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int r = (int)(e / n), c = (int)(e % n);
+        if (r < nao && c < nao) continue;
+        S[mat0 + e] = (r == c) ? 1.0 : 0.0;
+        H0[mat0 + e] = 0.0;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ES2 shell-resolved Coulomb matrix (secondorder.py:799-870, average.py:41-56)
+//   gamma_ij = 1 / sqrt(d_AB + eta_ij^-2),  eta_ij = 2 / (1/eta_i + 1/eta_j)
+//   d_AB = (R_AB + eps)^2 off-site, eps on-site.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_gamma(xtb_batch b, xtb_params par,
+                                               const double* __restrict__ pos,
+                                               double* __restrict__ gam) {
+    const int* tl = b.tile_sh + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int i = tl[1] + threadIdx.x / 16;
+    const int j = tl[2] + threadIdx.x % 16;
+    const int nsh = b.mol_nsh[m];
+    if (i >= nsh || j >= nsh || j < i) return;
+    const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
+    const double eps = 2.220446049250313e-16;
+    const int ga = at0 + b.sh_atom[sh0 + i], gb = at0 + b.sh_atom[sh0 + j];
+    double d;
+    if (ga == gb) d = eps;
+    else {
+        const double dx = pos[3 * ga] - pos[3 * gb], dy = pos[3 * ga + 1] - pos[3 * gb + 1],
+                     dz = pos[3 * ga + 2] - pos[3 * gb + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz) + eps;
+        d = r * r;
+    }
+    const double hi = par.us_hard[b.sh_us[sh0 + i]] + eps;
+    const double hj = par.us_hard[b.sh_us[sh0 + j]] + eps;
+    const double avg = 2.0 / (1.0 / hi + 1.0 / hj);
+    const double g = 1.0 / sqrt(d + 1.0 / (avg * avg));
+    const size_t g0 = (size_t)b.mol_gam0[m];
+    gam[g0 + (size_t)i * nsh + j] = g;
+    gam[g0 + (size_t)j * nsh + i] = g;
+}
+
+// ---------------------------------------------------------------------------
+// repulsion energy (repulsion/base.py:269-334, rep.py:65-98):
+//   E = sum_{A<B, R<=25} Z_A Z_B exp(-sqrt(a_A a_B) R^kexp) / R
+// ---------------------------------------------------------------------------
+__global__ void k_zero_rep(xtb_batch b, double* out) {
+    const int m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (m < b.nmol) out[(size_t)m * XTB_OUT_STRIDE + XTB_OUT_EREP] = 0.0;
+}
+
+__global__ void __launch_bounds__(256) k_repulsion(xtb_batch b, xtb_params par,
+                                                   const double* __restrict__ pos, double* out) {
+    const int* tl = b.tile_at + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int A = tl[1] + threadIdx.x / 16;
+    const int B = tl[2] + threadIdx.x % 16;
+    const int nat = b.mol_nat[m];
+    double e = 0.0;
+    if (A < nat && B < nat && B > A) {
+        const int at0 = b.mol_atom0[m];
+        const int ga = at0 + A, gb = at0 + B;
+        const double dx = pos[3 * ga] - pos[3 * gb], dy = pos[3 * ga + 1] - pos[3 * gb + 1],
+                     dz = pos[3 * ga + 2] - pos[3 * gb + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz);
+        if (r <= par.rep_cutoff) {
+            const int sa = b.atom_sp[ga], sb = b.atom_sp[gb];
+            const double alpha = sqrt(par.sp_arep[sa] * par.sp_arep[sb] + 2.2250738585072014e-308);
+            e = par.sp_zeff[sa] * par.sp_zeff[sb] * exp(-alpha * pow(r, par.kexp)) / r;
+        }
+    }
+    // one molecule per tile: reduce in the workgroup, one atomic per tile
+    __shared__ double red[4];
+    double v = e;
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        atomicAdd(out + (size_t)m * XTB_OUT_STRIDE + XTB_OUT_EREP, red[0] + red[1] + red[2] + red[3]);
+}
+
+}  // namespace xtb
+
+using namespace xtb;
+
+static int check_launch() { return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP; }
+
+extern "C" int32_t xtb_abi_version(void) { return XTB_ABI_VERSION; }
+
+extern "C" int32_t xtb_abi_struct_sizes(int64_t* out4) {
+    if (!out4) return XTB_EINVAL;
+    out4[0] = sizeof(xtb_batch);
+    out4[1] = sizeof(xtb_params);
+    out4[2] = sizeof(xtb_scf_opts);
+    out4[3] = sizeof(xtb_scf_state);
+    return XTB_OK;
+}
+
+extern "C" int32_t xtb_workspace_doubles(const xtb_batch* batch, int32_t generations,
+                                         int64_t* out4) {
+    if (!batch || !out4 || generations < 0) return XTB_EINVAL;
+    out4[0] = batch->nmat_total;
+    out4[1] = batch->ngam_total;
+    out4[2] = batch->nvec_total;
+    out4[3] = (int64_t)(generations + 1) * batch->nvec_total;
+    return XTB_OK;
+}
+
+extern "C" int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
+                                  const double* positions, double* S, double* H0, double* cn,
+                                  void* stream) {
+    if (!batch || !par || !positions || !S || !H0 || !cn) return XTB_EINVAL;
+    if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
+    if (par->nus > XTB_MAX_US_LDS) return XTB_ETOOBIG;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_cn, dim3((batch->nat_total + 255) / 256), dim3(256), 0, s, *batch, *par,
+                       positions, cn);
+    if (batch->ntile_sh > 0)
+        hipLaunchKernelGGL(k_s_h0, dim3(batch->ntile_sh), dim3(256), 0, s, *batch, *par, positions,
+                           cn, S, H0);
+    // padding orbitals (padded layout); molecules without padding return at once
+    hipLaunchKernelGGL(k_pad_fill, dim3(8, batch->nmol), dim3(256), 0, s, *batch, S, H0);
+    return check_launch();
+}
+
+extern "C" int32_t xtb_build_gamma(const xtb_batch* batch, const xtb_params* par,
+                                   const double* positions, double* gamma, void* stream) {
+    if (!batch || !par || !positions || !gamma) return XTB_EINVAL;
+    if (batch->nmol <= 0) return XTB_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    if (batch->ntile_sh > 0)
+        hipLaunchKernelGGL(k_gamma, dim3(batch->ntile_sh), dim3(256), 0, s, *batch, *par, positions,
+                           gamma);
+    return check_launch();
+}
+
+extern "C" int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
+                                 const double* positions, double* out, void* stream) {
+    if (!batch || !par || !positions || !out) return XTB_EINVAL;
+    if (batch->nmol <= 0) return XTB_EINVAL;
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_zero_rep, dim3((batch->nmol + 255) / 256), dim3(256), 0, s, *batch, out);
+    if (batch->ntile_at > 0)
+        hipLaunchKernelGGL(k_repulsion, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
+                           positions, out);
+    return check_launch();
+}

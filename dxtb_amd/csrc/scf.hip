@@ -1,0 +1,595 @@
+// Persistent per-molecule GFN1 SCF on gfx950: one workgroup owns one
+// molecule and runs its whole self-consistent cycle without returning to
+// the host.  Follows upstream's default ("full") SCF in potential mode
+// (src/dxtb/_src/scf/unrolling/default.py:71-140, scf/base.py:651-907):
+//
+//   v -> F = H0 - 1/2 S o (v_i + v_j)           (Fock build)
+//     -> F C = S C e                             (generalized eigensolve)
+//     -> f = Fermi(e), P = C diag(f) C^T         (smearing, density)
+//     -> q = n0 - diag(P S), e_orb = diag(P H0)  (Mulliken)
+//     -> v' = spread(gamma q_sh + hd q_at^2)     (ES2 + ES3 potential)
+//   mixed with upstream's Anderson mixer (scf/mixer/anderson.py:163-317).
+//
+// Eigensolver: the generalized problem is reduced once per geometry with the
+// Loewdin orthogonaliser X = S^-1/2 (itself from a Jacobi solve of S).  Every
+// SCF step transforms F into the basis of the previous step's orbitals,
+// A = C^T F C, which is close to diagonal, and finishes it with a few sweeps
+// of parallel-ordered cyclic Jacobi whose rotations are applied directly to
+// the stored orbitals.  A lives in LDS (packed upper triangle) whenever it
+// fits, and so do the orbitals when both fit.
+#include "common.h"
+#include "wg.h"
+
+namespace xtb {
+
+struct Lay {
+    // LDS layout in doubles (offsets), identical on host and device
+    int red, cs, skey, sidx, ev, ftot, qsh, vsh, gq, qat, big, total_fixed;
+};
+
+__host__ __device__ inline int next_pow2(int v) {
+    int p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+__host__ __device__ inline Lay make_lay(int nmax, int max_nsh, int max_nat) {
+    Lay L;
+    const int npow = next_pow2(nmax < 2 ? 2 : nmax);
+    int o = 0;
+    L.red = o; o += 256;
+    L.cs = o; o += 2 * (nmax / 2 + 2);
+    L.skey = o; o += npow;
+    L.sidx = o; o += (npow + 1) / 2;  // ints
+    L.ev = o; o += nmax;
+    L.ftot = o; o += nmax;
+    L.qsh = o; o += max_nsh;
+    L.vsh = o; o += max_nsh;
+    L.gq = o; o += max_nsh;
+    L.qat = o; o += max_nat;
+    o = (o + 1) & ~1;
+    L.big = o;
+    L.total_fixed = o;
+    return L;
+}
+
+struct Mol {
+    int m, n, nao, nsh, nat, vec0, sh0, at0;
+    size_t mat0, gam0;
+    const double *S, *H0, *gam;
+    double *F, *Ct, *T, *P;
+    double *pot, *vnew, *q, *eorb, *emo, *occ, *qsh_out;
+    double nel_a, nel_b;
+    int nvec_total;
+};
+
+struct Shared {
+    double *red, *cs, *skey, *ev, *ftot, *qsh, *vsh, *gq, *qat, *big;
+    int* sidx;
+    int npow, big_cap;
+};
+
+// ---------------------------------------------------------------------------
+// eigen step: A (full, in mol.F) -> eigenvalues in sh.ev, rotations applied
+// to mol.Ct.  Returns Jacobi sweeps (-1 on failure).
+// ---------------------------------------------------------------------------
+__device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_sweeps, double rel_tol) {
+    const int n = M.n;
+    // scale for the absolute tolerance
+    double dmax = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, fabs(M.F[(size_t)i * n + i]));
+    dmax = wg_max(dmax, sh.red);
+    const double tol = rel_tol * fmax(dmax, 1.0);
+    const int npk = n * (n + 1) / 2;
+    int sweeps;
+    if (npk <= sh.big_cap) {
+        SymPacked A{sh.big, n};
+        for (int idx = threadIdx.x; idx < n * n; idx += NT) {
+            const int i = idx / n, j = idx % n;
+            if (j >= i) A.at(i, j) = M.F[(size_t)i * n + j];
+        }
+        const bool cl = (npk + n * n) <= sh.big_cap;
+        double* cbuf = sh.big + npk;
+        if (cl) {
+            for (int idx = threadIdx.x; idx < n * n; idx += NT) cbuf[idx] = M.Ct[idx];
+        }
+        __syncthreads();
+        if (cl) sweeps = wg_jacobi(A, n, RowsLds{cbuf, n}, max_sweeps, tol, sh.cs, sh.red);
+        else sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red);
+        for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = A.at(i, i);
+        if (cl) {
+            for (int idx = threadIdx.x; idx < n * n; idx += NT) M.Ct[idx] = cbuf[idx];
+        }
+    } else {
+        SymFull A{M.F, n};
+        sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red);
+        for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = A.at(i, i);
+    }
+    __syncthreads();
+    return sweeps;
+}
+
+// Loewdin orthogonaliser: Ct <- X = S^-1/2 (symmetric, so rows == columns).
+__device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* status) {
+    const int n = M.n;
+    const size_t nn = (size_t)n * n;
+    for (size_t e = threadIdx.x; e < nn; e += NT) {
+        M.F[e] = M.S[e];
+        const int r = (int)(e / n), c = (int)(e % n);
+        M.Ct[e] = (r == c) ? 1.0 : 0.0;
+    }
+    __syncthreads();
+    int sw = eig_step(M, sh, 60, 1e-15);
+    if (sw < 0) *status |= XTB_MOL_JACOBI;
+    // T[k][mu] = s_k^-1/2 U[mu][k]
+    double bad = 0.0;
+    for (size_t e = threadIdx.x; e < nn; e += NT) {
+        const int k = (int)(e / n);
+        const double s = sh.ev[k];
+        if (!(s > 0.0)) bad = 1.0;
+        M.T[e] = M.Ct[e] / sqrt(s);
+    }
+    bad = wg_max(bad, sh.red);
+    if (bad > 0.0) *status |= XTB_MOL_NONFINITE;
+    // X = U s^-1/2 U^T  ->  F
+    wg_gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
+    for (size_t e = threadIdx.x; e < nn; e += NT) M.Ct[e] = M.F[e];
+    __syncthreads();
+    return sw < 0 ? 0 : sw;
+}
+
+// ---------------------------------------------------------------------------
+// Fermi smearing of one spin channel (src/dxtb/_src/wavefunction/
+// filling.py:201-366): mid-gap start, Newton on the electron count, exponent
+// cutoff 50, threshold on |N - sum f|.  Sorted eigenvalues in sh.skey.
+// Writes the sorted occupations to occ_out and adds them to sh.ftot.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, double nel, double kt, double thr,
+                              int maxiter, bool padded, double* occ_out, double* efermi) {
+    const int n = M.n;
+    const double eps = 2.220446049250313e-16;
+    if (fabs(nel) < eps) {
+        for (int j = threadIdx.x; j < n; j += NT) occ_out[j] = 0.0;
+        *efermi = 0.0;
+        __syncthreads();
+        return true;
+    }
+    int homo = (int)ceil(nel - 5e-15) - 1;
+    if (homo < 0) homo = 0;
+    if (homo > n - 1) homo = n - 1;
+    const bool lumo_missing = (n - 1 <= homo);
+    const int lumo = lumo_missing ? homo : homo + 1;
+    double ef = 0.5 * (sh.skey[homo] + sh.skey[lumo]);
+    if (padded && (homo >= M.nao || lumo >= M.nao)) ef *= 2.0;  // filling.py:254-264
+    const double target = (double)(homo + 1);
+    for (int it = 0; it < maxiter; ++it) {
+        double acc[2] = {0.0, 0.0};
+        for (int j = threadIdx.x; j < n; j += NT) {
+            const double x = (sh.skey[j] - ef) / kt;
+            if (x < 50.0) {
+                const double et = exp(x);
+                acc[0] += 1.0 / (et + 1.0);
+                acc[1] += et / (kt * (et + 1.0) * (et + 1.0));
+            } else {
+                acc[1] += eps;
+            }
+        }
+        wg_sum_n<2>(acc, sh.red);
+        const double diff = target - acc[0];
+        if (fabs(diff) <= thr) {
+            for (int j = threadIdx.x; j < n; j += NT) {
+                const double x = (sh.skey[j] - ef) / kt;
+                const double f = (x < 50.0) ? 1.0 / (exp(x) + 1.0) : 0.0;
+                occ_out[j] = f;
+                sh.ftot[sh.sidx[j]] += f;
+            }
+            *efermi = ef;
+            __syncthreads();
+            return true;
+        }
+        ef += diff / acc[1];
+    }
+    for (int j = threadIdx.x; j < n; j += NT) occ_out[j] = 0.0;
+    *efermi = ef;
+    __syncthreads();
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// one SCF map v_in -> v_out (scf/base.py:651-675 iterate_potential);
+// leaves P, q, e_orb, sorted e/occupations and shell/atom charges behind.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
+                        const xtb_params& par, const xtb_scf_opts& o, const double* vin,
+                        double* vout, int* status, double* ef) {
+    const int n = M.n;
+    const size_t nn = (size_t)n * n;
+    // Fock build (scf/base.py:817-859)
+    for (size_t e = threadIdx.x; e < nn; e += NT) {
+        const int r = (int)(e / n), c = (int)(e % n);
+        M.F[e] = M.H0[e] - 0.5 * M.S[e] * (vin[r] + vin[c]);
+    }
+    __syncthreads();
+    // T = F C ; A = C^T T  -> F
+    wg_gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
+    wg_gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
+    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol);
+    if (sw < 0) *status |= XTB_MOL_JACOBI;
+    // sort eigenvalues
+    for (int j = threadIdx.x; j < sh.npow; j += NT) {
+        sh.skey[j] = (j < n) ? sh.ev[j] : INFINITY;
+        sh.sidx[j] = j;
+    }
+    for (int j = threadIdx.x; j < n; j += NT) sh.ftot[j] = 0.0;
+    __syncthreads();
+    wg_bitonic_sort(sh.skey, sh.sidx, sh.npow);
+    // Fermi occupations, alpha then beta
+    bool ok = fermi_channel(M, sh, M.nel_a, o.kt, o.fermi_thresh, o.fermi_maxiter, o.padded != 0,
+                            M.occ, &ef[0]);
+    ok = fermi_channel(M, sh, M.nel_b, o.kt, o.fermi_thresh, o.fermi_maxiter, o.padded != 0,
+                       M.occ + M.nvec_total, &ef[1]) && ok;
+    if (!ok) *status |= XTB_MOL_FERMI;
+    // P = C diag(f) C^T
+    for (size_t e = threadIdx.x; e < nn; e += NT) M.T[e] = sh.ftot[e / n] * M.Ct[e];
+    __syncthreads();
+    wg_gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.P, n, sh.big);
+    // Mulliken charges and orbital energies (scf/base.py:764-815), one wave per row
+    {
+        const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+        for (int i = w; i < n; i += NWAVE) {
+            double sq = 0.0, se = 0.0;
+            if (i < M.nao) {
+                const double* pr = M.P + (size_t)i * n;
+                const double* sr = M.S + (size_t)i * n;
+                const double* hr = M.H0 + (size_t)i * n;
+                for (int j = l; j < M.nao; j += WAVE) {
+                    sq = fma(pr[j], sr[j], sq);
+                    se = fma(pr[j], hr[j], se);
+                }
+            }
+            sq = wave_sum(sq);
+            se = wave_sum(se);
+            if (l == 0) {
+                M.q[i] = (i < M.nao) ? b.n0[M.vec0 + i] - sq : 0.0;
+                M.eorb[i] = (i < M.nao) ? se : 0.0;
+            }
+        }
+    }
+    __syncthreads();
+    // shell and atom charges
+    for (int s = threadIdx.x; s < M.nsh; s += NT) {
+        const int o0 = b.sh_ao[M.sh0 + s];
+        const int nl = 2 * par.us_l[b.sh_us[M.sh0 + s]] + 1;
+        double acc = 0.0;
+        for (int x = 0; x < nl; ++x) acc += M.q[o0 + x];
+        sh.qsh[s] = acc;
+    }
+    __syncthreads();
+    for (int a = threadIdx.x; a < M.nat; a += NT) {
+        const int s0 = b.atom_sh0[M.at0 + a], ns = b.atom_nsh[M.at0 + a];
+        double acc = 0.0;
+        for (int x = 0; x < ns; ++x) acc += sh.qsh[s0 + x];
+        sh.qat[a] = acc;
+    }
+    __syncthreads();
+    // ES2 (gamma q_sh) + ES3 (hd q_at^2) shell potential, one wave per row
+    {
+        const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
+        for (int s = w; s < M.nsh; s += NWAVE) {
+            const double* gr = M.gam + (size_t)s * M.nsh;
+            double acc = 0.0;
+            for (int t = l; t < M.nsh; t += WAVE) acc = fma(gr[t], sh.qsh[t], acc);
+            acc = wave_sum(acc);
+            if (l == 0) {
+                const int a = b.sh_atom[M.sh0 + s];
+                const double hd = par.sp_gam3[b.atom_sp[M.at0 + a]];
+                sh.gq[s] = acc;
+                sh.vsh[s] = acc + hd * sh.qat[a] * sh.qat[a];
+            }
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const int s = (i < M.nao) ? b.ao_sh[M.vec0 + i] : -1;
+        vout[i] = (s >= 0) ? sh.vsh[s] : 0.0;
+    }
+    __syncthreads();
+    return sw < 0 ? 0 : sw;
+}
+
+// ---------------------------------------------------------------------------
+// Anderson mixing with soft start (scf/mixer/anderson.py:163-317), history in
+// a ring buffer of G = generations+1 slots.  Returns the convergence flag of
+// the step (mixer/base.py:229-256 on the step's residual).
+// ---------------------------------------------------------------------------
+struct Mixer {
+    int step, head, G;
+};
+
+__device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh, const xtb_scf_opts& o,
+                         double* hx, double* hf, const double* xnew, const double* xold,
+                         double* xout) {
+    const int n = M.n, G = mx.G;
+    auto X = [&](int h) { return hx + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
+    auto Fh = [&](int h) { return hf + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
+    if (mx.step == 0) {
+        double* x0 = X(0);
+        for (int i = threadIdx.x; i < n; i += NT) x0[i] = xold[i];
+        __syncthreads();
+    }
+    mx.step += 1;
+    const int gen = G - 1;
+    double* x0 = X(0);
+    double* f0 = Fh(0);
+    double l2 = 0.0, linf = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double f = xnew[i] - x0[i];
+        f0[i] = f;
+        l2 = fma(f, f, l2);
+        linf = fmax(linf, fabs(f));
+    }
+    l2 = wg_sum(l2, sh.red);
+    linf = wg_max(linf, sh.red);
+    const bool conv = (sqrt(l2) < o.x_atol) && (linf < o.x_atol_max);
+    double theta[5] = {0, 0, 0, 0, 0};
+    int nn = 0;
+    if (mx.step > gen) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
+    if (nn > 0) {
+        // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn (nn <= 5)
+        double acc[20];
+#pragma unroll
+        for (int k = 0; k < 20; ++k) acc[k] = 0.0;
+        for (int i = threadIdx.x; i < n; i += NT) {
+            double d[5];
+            const double fi = f0[i];
+#pragma unroll
+            for (int h = 0; h < 5; ++h) d[h] = (h < nn) ? fi - Fh(h + 1)[i] : 0.0;
+            int k = 0;
+#pragma unroll
+            for (int h = 0; h < 5; ++h)
+#pragma unroll
+                for (int j = h; j < 5; ++j) acc[k++] += d[h] * d[j];
+#pragma unroll
+            for (int h = 0; h < 5; ++h) acc[15 + h] += d[h] * fi;
+        }
+        wg_sum_n<20>(acc, sh.red);
+        // solve (a * diag(1 + off^2)) theta = b by Gaussian elimination with
+        // partial pivoting; one thread, system held in LDS
+        double* a = sh.red;          // [5][6]
+        double* th = sh.red + 32;    // [5]
+        if (threadIdx.x == 0) {
+            int k = 0;
+#pragma unroll
+            for (int h = 0; h < 5; ++h)
+#pragma unroll
+                for (int j = h; j < 5; ++j) { a[h * 6 + j] = acc[k]; a[j * 6 + h] = acc[k]; ++k; }
+#pragma unroll
+            for (int h = 0; h < 5; ++h) a[h * 6 + 5] = acc[15 + h];
+            for (int h = 0; h < nn; ++h) a[h * 6 + h] *= 1.0 + o.diag_offset * o.diag_offset;
+            for (int c = 0; c < nn; ++c) {
+                int piv = c;
+                for (int r = c + 1; r < nn; ++r)
+                    if (fabs(a[r * 6 + c]) > fabs(a[piv * 6 + c])) piv = r;
+                if (piv != c)
+                    for (int x = 0; x < 6; ++x) {
+                        const double t = a[c * 6 + x];
+                        a[c * 6 + x] = a[piv * 6 + x];
+                        a[piv * 6 + x] = t;
+                    }
+                for (int r = c + 1; r < nn; ++r) {
+                    const double fct = a[r * 6 + c] / a[c * 6 + c];
+                    for (int x = c; x < 6; ++x) a[r * 6 + x] -= fct * a[c * 6 + x];
+                }
+            }
+            for (int r = nn - 1; r >= 0; --r) {
+                double sacc = a[r * 6 + 5];
+                for (int x = r + 1; x < nn; ++x) sacc -= a[r * 6 + x] * th[x];
+                th[r] = sacc / a[r * 6 + r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 5; ++h) theta[h] = (h < nn) ? th[h] : 0.0;
+        __syncthreads();
+    }
+    // new slot for the mixed vector = oldest slot
+    const int newhead = (mx.head - 1 + G) % G;
+    double* xn = hx + (size_t)newhead * M.nvec_total + M.vec0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        double xm;
+        if (nn > 0) {
+            double xb = x0[i], fb = f0[i];
+#pragma unroll
+            for (int h = 0; h < 5; ++h) {
+                if (h < nn) {
+                    const double xh = X(h + 1)[i], fh = Fh(h + 1)[i];
+                    xb += theta[h] * (xh - x0[i]);
+                    fb -= theta[h] * (f0[i] - fh);
+                }
+            }
+            xm = xb + o.damp * fb;
+        } else {
+            xm = x0[i] + o.damp_init * f0[i];
+        }
+        xout[i] = xm;
+        xn[i] = xm;  // slot newhead may alias X(G-1): element-wise read-before-write
+    }
+    __syncthreads();
+    mx.head = newhead;
+    return conv;
+}
+
+__global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const double* S,
+                                            const double* H0, const double* gam,
+                                            xtb_scf_opts o, xtb_scf_state st, int big_cap) {
+    extern __shared__ double lds[];
+    const Lay L = make_lay(b.max_n, b.max_nsh, b.max_nat);
+    Shared sh;
+    sh.red = lds + L.red;
+    sh.cs = lds + L.cs;
+    sh.skey = lds + L.skey;
+    sh.sidx = reinterpret_cast<int*>(lds + L.sidx);
+    sh.ev = lds + L.ev;
+    sh.ftot = lds + L.ftot;
+    sh.qsh = lds + L.qsh;
+    sh.vsh = lds + L.vsh;
+    sh.gq = lds + L.gq;
+    sh.qat = lds + L.qat;
+    sh.big = lds + L.big;
+    sh.npow = next_pow2(b.max_n < 2 ? 2 : b.max_n);
+    sh.big_cap = big_cap;
+
+    Mol M;
+    M.m = st.order[blockIdx.x];
+    M.n = b.mol_n[M.m];
+    M.nao = b.mol_nao[M.m];
+    M.nsh = b.mol_nsh[M.m];
+    M.nat = b.mol_nat[M.m];
+    M.vec0 = b.mol_vec0[M.m];
+    M.sh0 = b.mol_sh0[M.m];
+    M.at0 = b.mol_atom0[M.m];
+    M.mat0 = (size_t)b.mol_mat0[M.m];
+    M.gam0 = (size_t)b.mol_gam0[M.m];
+    M.S = S + M.mat0;
+    M.H0 = H0 + M.mat0;
+    M.gam = gam + M.gam0;
+    M.F = st.F + M.mat0;
+    M.Ct = st.C + M.mat0;
+    M.T = st.T + M.mat0;
+    M.P = st.P + M.mat0;
+    M.pot = st.pot + M.vec0;
+    M.vnew = st.vnew + M.vec0;
+    M.q = st.q + M.vec0;
+    M.eorb = st.eorb + M.vec0;
+    M.emo = st.emo + M.vec0;
+    M.occ = st.occ + M.vec0;
+    M.qsh_out = st.qsh + M.sh0;
+    M.nel_a = b.mol_nel[2 * M.m];
+    M.nel_b = b.mol_nel[2 * M.m + 1];
+    M.nvec_total = b.nvec_total;
+    double* out = st.out + (size_t)M.m * XTB_OUT_STRIDE;
+    if (M.n <= 0) {
+        if (threadIdx.x < XTB_OUT_STRIDE && threadIdx.x != XTB_OUT_EREP) out[threadIdx.x] = 0.0;
+        return;
+    }
+    int status = 0;
+    int sweeps = ortho_setup(M, sh, &status);
+    double ef[2] = {0.0, 0.0};
+
+    // guess: zero charges -> zero potential (scf/base.py:363-420)
+    for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = 0.0;
+    __syncthreads();
+    Mixer mx{0, 0, o.generations + 1};
+    int iters = 0;
+    // unrolling/default.py:80-100: evaluate the guess, optionally mix it
+    sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef);
+    ++iters;
+    if (o.mix_guess) {
+        anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot);
+    } else {
+        for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = M.vnew[i];
+        __syncthreads();
+    }
+    bool converged = false;
+    for (int it = 0; it < o.maxiter; ++it) {
+        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef);
+        ++iters;
+        const bool c = anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot);
+        if (c) { converged = true; break; }
+    }
+    if (!converged && o.maxiter > 0) status |= XTB_MOL_MAXITER;
+    if (o.maxiter == 0) converged = true;
+    // converged_to_charges (scf/base.py:472-513): density of the last SCF
+    // output potential; the potential of its charges goes to st.pot
+    sweeps += scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef);
+
+    // energies (scf/base.py:514-623)
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < M.nao; i += NT) acc[0] += M.eorb[i];
+    for (int s = threadIdx.x; s < M.nsh; s += NT) {
+        acc[1] += 0.5 * sh.qsh[s] * sh.gq[s];
+        M.qsh_out[s] = sh.qsh[s];
+    }
+    for (int a = threadIdx.x; a < M.nat; a += NT) {
+        const double hd = par.sp_gam3[b.atom_sp[M.at0 + a]];
+        acc[2] += hd * sh.qat[a] * sh.qat[a] * sh.qat[a] / 3.0;
+    }
+    {
+        const double eps = 2.220446049250313e-16;
+        for (int c = 0; c < 2; ++c)
+            for (int j = threadIdx.x; j < M.n; j += NT) {
+                const double f = M.occ[c * M.nvec_total + j];
+                const double f1 = fmax(f, eps), f2 = fmax(1.0 - f, eps);
+                acc[3] += f1 * log(f1) + f2 * log(f2);
+            }
+    }
+    wg_sum_n<4>(acc, sh.red);
+    for (int j = threadIdx.x; j < M.n; j += NT) M.emo[j] = sh.skey[j];
+    if (o.want_w) {
+        // W = C diag(f e) C^T (analytical.py:186-190)
+        const size_t nn = (size_t)M.n * M.n;
+        for (size_t e = threadIdx.x; e < nn; e += NT) {
+            const int k = (int)(e / M.n);
+            M.F[e] = sh.ftot[k] * sh.ev[k] * M.Ct[e];
+        }
+        __syncthreads();
+        wg_gemm<true, false>(M.n, M.n, M.n, 1.0, M.Ct, M.n, M.F, M.n, 0.0, M.T, M.n, sh.big);
+    }
+    if (threadIdx.x == 0) {
+        const double escf = acc[0] + acc[1] + acc[2];
+        if (!isfinite(escf) || !isfinite(acc[3])) status |= XTB_MOL_NONFINITE;
+        out[XTB_OUT_ESCF] = escf;
+        out[XTB_OUT_EH0] = acc[0];
+        out[XTB_OUT_EES2] = acc[1];
+        out[XTB_OUT_EES3] = acc[2];
+        out[XTB_OUT_FENERGY] = acc[3] * o.kt;
+        out[XTB_OUT_ITER] = (double)iters;
+        out[XTB_OUT_STATUS] = (double)status;
+        out[XTB_OUT_EFERMI_A] = ef[0];
+        out[XTB_OUT_EFERMI_B] = ef[1];
+        out[XTB_OUT_SWEEPS] = (double)sweeps;
+    }
+}
+
+}  // namespace xtb
+
+using namespace xtb;
+
+extern "C" int32_t xtb_scf_lds_bytes(const xtb_batch* batch, int32_t* bytes, int32_t* big_cap) {
+    if (!batch || !bytes || !big_cap) return XTB_EINVAL;
+    const Lay L = make_lay(batch->max_n, batch->max_nsh, batch->max_nat);
+    const int budget = 160 * 1024 / 8;
+    const int n = batch->max_n;
+    const int want_c = n * (n + 1) / 2 + n * n;
+    const int want_a = n * (n + 1) / 2;
+    const int avail = budget - L.total_fixed;
+    int cap = NGEMM_LDS;
+    if (want_c <= avail) cap = want_c > cap ? want_c : cap;
+    else if (want_a <= avail) cap = want_a > cap ? want_a : cap;
+    if (L.total_fixed + cap > budget) return XTB_ETOOBIG;
+    *big_cap = cap;
+    *bytes = (L.total_fixed + cap) * 8;
+    return XTB_OK;
+}
+
+extern "C" int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par, const double* S,
+                           const double* H0, const double* gamma, const xtb_scf_opts* opts,
+                           const xtb_scf_state* state, void* stream) {
+    if (!batch || !par || !S || !H0 || !gamma || !opts || !state) return XTB_EINVAL;
+    if (batch->nmol <= 0 || batch->max_n <= 0) return XTB_EINVAL;
+    if (batch->max_n > 1024) return XTB_ETOOBIG;
+    if (opts->generations < 1 || opts->generations > 5) return XTB_EINVAL;
+    if (!state->F || !state->C || !state->T || !state->P || !state->pot || !state->vnew ||
+        !state->q || !state->eorb || !state->emo || !state->occ || !state->hist_x ||
+        !state->hist_f || !state->qsh || !state->out || !state->order)
+        return XTB_EINVAL;
+    int32_t bytes = 0, cap = 0;
+    int rc = xtb_scf_lds_bytes(batch, &bytes, &cap);
+    if (rc != XTB_OK) return rc;
+    if (hipFuncSetAttribute((const void*)k_scf, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
+        hipSuccess)
+        return XTB_EHIP;
+    hipLaunchKernelGGL(k_scf, dim3(batch->nmol), dim3(NT), bytes, (hipStream_t)stream, *batch, *par,
+                       S, H0, gamma, *opts, *state, cap);
+    return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP;
+}

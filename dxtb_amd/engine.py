@@ -1,0 +1,310 @@
+"""Ragged batch descriptor and the native GFN1 pipeline.
+
+A *batch* is a list of molecules, each given by its real atoms (atomic
+numbers > 0) and positions in bohr.  ``BatchLayout`` builds, with vectorised
+numpy on the host, the index maps the kernels need (atom -> shell ->
+orbital, offsets of every molecule's packed matrices, tile lists of the
+shell-pair and atom-pair spaces).  In the *ragged* layout each molecule's
+matrices are n_b x n_b with n_b its own orbital count; in the *padded*
+layout every molecule gets the batch-maximum dimension and its padding
+orbitals carry a unit overlap diagonal and zero Hamiltonian, which is what
+upstream's padded batch computes (src/dxtb/_src/scf/unrolling/base.py:122-139).
+
+``Pipeline.run`` then calls, on one HIP stream, the entry points of
+include/xtb_hip.h: S/H0/CN, gamma, repulsion, the persistent SCF and
+optionally the analytic gradient.
+"""
+
+from __future__ import annotations
+
+import ctypes as ct
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from . import _lib
+from . import constants as C
+from .param import SpeciesTables
+
+MAT_ALIGN = 32  # doubles (256 bytes) between molecules' matrices
+
+
+def _cumsum0(a: np.ndarray) -> np.ndarray:
+    out = np.zeros(len(a) + 1, dtype=np.int64)
+    np.cumsum(a, out=out[1:])
+    return out
+
+
+def alpha_beta(nel: np.ndarray, spin) -> tuple[np.ndarray, np.ndarray]:
+    """Alpha/beta electrons (src/dxtb/_src/wavefunction/filling.py:41-113),
+    rounded as the occupation sum is upstream (scf/base.py:880)."""
+    nel = np.asarray(nel, dtype=np.float64)
+    uhf = np.zeros_like(nel) if spin is None else np.broadcast_to(np.asarray(spin, dtype=np.float64), nel.shape)
+    nr = np.round(nel)
+    nuhf = np.where(np.remainder(uhf, 2) == np.remainder(nr, 2), uhf, np.remainder(nr, 2))
+    diff = np.minimum(nuhf, nel)
+    nb = (nel - diff) / 2.0
+    na = nb + diff
+    return np.round(na), np.round(nb)
+
+
+def _tiles(counts: np.ndarray) -> np.ndarray:
+    """Upper-triangle 16x16 tiles of every molecule's pair space: rows (mol, i0, j0)."""
+    out = []
+    for m, c in enumerate(counts):
+        t = (int(c) + 15) // 16
+        if t == 0:
+            continue
+        iu, ju = np.triu_indices(t)
+        blk = np.empty((len(iu), 3), dtype=np.int32)
+        blk[:, 0] = m
+        blk[:, 1] = iu * 16
+        blk[:, 2] = ju * 16
+        out.append(blk)
+    if not out:
+        return np.zeros((0, 3), dtype=np.int32)
+    return np.concatenate(out)
+
+
+@dataclass
+class BatchLayout:
+    numbers: list  # list of int arrays (real atoms)
+    tables: SpeciesTables
+    padded: bool
+    chrg: np.ndarray
+    spin: object = None
+    arrays: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        tb = self.tables
+        mols = [np.asarray(z, dtype=np.int64) for z in self.numbers]
+        B = len(mols)
+        nat = np.array([len(z) for z in mols], dtype=np.int64)
+        if B == 0 or np.any(nat == 0):
+            raise ValueError("every molecule needs at least one atom")
+        Z = np.concatenate(mols)
+        sp = tb.sp_of_z[Z]
+        if np.any(sp < 0):
+            raise ValueError("element outside the species tables")
+        atom0 = _cumsum0(nat)
+        atom_mol = np.repeat(np.arange(B), nat)
+        atom_nsh = tb.nsh_of_sp[sp].astype(np.int64)
+        nsh = np.add.reduceat(atom_nsh, atom0[:-1])
+        sh0 = _cumsum0(nsh)
+        first_sh = _cumsum0(atom_nsh)[:-1]
+        sh_atom_g = np.repeat(np.arange(len(Z)), atom_nsh)
+        k_in_atom = np.arange(sh0[-1]) - first_sh[sh_atom_g]
+        sh_us = tb.us_base[sp[sh_atom_g]] + k_in_atom
+        sh_mol = atom_mol[sh_atom_g]
+        sh_size = 2 * tb.us_l[sh_us].astype(np.int64) + 1
+        nao = np.add.reduceat(sh_size, sh0[:-1])
+        sh_start_g = _cumsum0(sh_size)[:-1]
+        ao0_real = _cumsum0(nao)
+        sh_ao = sh_start_g - ao0_real[sh_mol]
+        n = np.full(B, nao.max()) if self.padded else nao.copy()
+        vec0 = _cumsum0(n)
+        ao_sh = np.full(vec0[-1], -1, dtype=np.int64)
+        n0 = np.zeros(vec0[-1])
+        ao_shell_g = np.repeat(np.arange(sh0[-1]), sh_size)
+        ao_mol = sh_mol[ao_shell_g]
+        slot = vec0[ao_mol] + (np.arange(ao0_real[-1]) - ao0_real[ao_mol])
+        ao_sh[slot] = ao_shell_g - sh0[ao_mol]
+        n0[slot] = tb.us_refocc[sh_us[ao_shell_g]] / sh_size[ao_shell_g]
+        nel = np.add.reduceat(n0, vec0[:-1]) - self.chrg
+        na, nb = alpha_beta(nel, self.spin)
+        msz = ((n * n + MAT_ALIGN - 1) // MAT_ALIGN) * MAT_ALIGN
+        mat0 = _cumsum0(msz)
+        gam0 = _cumsum0(nsh * nsh)
+        self.B, self.nat, self.nsh, self.nao, self.n = B, nat, nsh, nao, n
+        self.atom0, self.sh0, self.vec0, self.mat0, self.gam0 = atom0, sh0, vec0, mat0, gam0
+        self.Z = Z
+        self.nel_a, self.nel_b = na, nb
+        self.sh_us = sh_us
+        self.ao_sh = ao_sh
+        i32 = np.int32
+        self.arrays = {
+            "mol_nat": nat.astype(i32), "mol_nsh": nsh.astype(i32), "mol_nao": nao.astype(i32),
+            "mol_n": n.astype(i32), "mol_atom0": atom0[:-1].astype(i32),
+            "mol_sh0": sh0[:-1].astype(i32), "mol_vec0": vec0[:-1].astype(i32),
+            "mol_mat0": mat0[:-1].astype(np.int64), "mol_gam0": gam0[:-1].astype(np.int64),
+            "mol_nel": np.stack([na, nb], axis=1).reshape(-1).astype(np.float64),
+            "atom_sp": sp.astype(i32), "atom_mol": atom_mol.astype(i32),
+            "atom_sh0": (first_sh - sh0[atom_mol]).astype(i32), "atom_nsh": atom_nsh.astype(i32),
+            "sh_atom": (sh_atom_g - atom0[sh_mol]).astype(i32), "sh_us": sh_us.astype(i32),
+            "sh_ao": sh_ao.astype(i32), "ao_sh": ao_sh.astype(i32), "n0": n0,
+            "tile_sh": _tiles(nsh).reshape(-1), "tile_at": _tiles(nat).reshape(-1),
+        }
+        self.ntile_sh = len(self.arrays["tile_sh"]) // 3
+        self.ntile_at = len(self.arrays["tile_at"]) // 3
+        self.nmat_total = int(mat0[-1])
+        self.ngam_total = int(gam0[-1])
+
+    @property
+    def nvec_total(self) -> int:
+        return int(self.vec0[-1])
+
+    def work_doubles(self) -> int:
+        """Device doubles held by one pipeline run (for admission control)."""
+        gen = 6
+        return int(7 * self.nmat_total + self.ngam_total + (9 + 2 * gen) * self.nvec_total
+                   + 8 * int(self.nat.sum()) + self.B * _lib.OUT_STRIDE)
+
+
+class DeviceTables:
+    """Parameter arrays of a SpeciesTables on the device + the ctypes struct."""
+
+    def __init__(self, tb: SpeciesTables, device):
+        f = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=device)
+        i = lambda a: torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+        self.t = {
+            "sp_rad": f(tb.sp_rad), "sp_rcov": f(tb.sp_rcov), "sp_gam3": f(tb.sp_gam3),
+            "sp_zeff": f(tb.sp_zeff), "sp_arep": f(tb.sp_arep), "us_level": f(tb.us_level),
+            "us_kcn": f(tb.us_kcn), "us_shpoly": f(tb.us_shpoly), "us_hard": f(tb.us_hard),
+            "us_l": i(tb.us_l), "us_nprim": i(tb.us_nprim), "us_alpha": f(tb.us_alpha),
+            "us_coeff": f(tb.us_coeff), "us_K": f(tb.us_K),
+        }
+        from .param import load_gfn1
+
+        kexp = load_gfn1()["repulsion"]["kexp"]
+        self.s = _lib.XtbParams(
+            nsp=len(tb.zs), nus=tb.nus, kexp=kexp, rep_cutoff=C.REPULSION_CUTOFF,
+            cn_cutoff=C.CN_CUTOFF, cn_steep=C.CN_STEEPNESS, int_cutoff=C.INTCUTOFF,
+            **{k: v.data_ptr() for k, v in self.t.items()},
+        )
+
+
+@dataclass
+class ScfOptions:
+    maxiter: int = 100
+    mix_guess: bool = True
+    generations: int = 5
+    fermi_maxiter: int = 200
+    fermi_etemp: float = 300.0
+    fermi_thresh: float | None = None
+    x_atol: float = 1e-4
+    x_atol_max: float = 1e-5
+    damp: float = 0.5
+    damp_init: float = 0.1
+    diag_offset: float = 0.01
+    jacobi_max_sweeps: int = 40
+    jacobi_tol: float = 1e-13
+
+    def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
+        thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
+        return _lib.XtbScfOpts(
+            maxiter=self.maxiter, mix_guess=int(self.mix_guess), generations=self.generations,
+            fermi_maxiter=self.fermi_maxiter, want_w=int(want_w),
+            jacobi_max_sweeps=self.jacobi_max_sweeps, padded=int(padded), pad0=0,
+            x_atol=self.x_atol, x_atol_max=self.x_atol_max, kt=self.fermi_etemp * C.KELVIN2AU,
+            damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
+            fermi_thresh=thr, jacobi_tol=self.jacobi_tol,
+        )
+
+
+class Pipeline:
+    """Device buffers for one batch and the calls into the HIP library."""
+
+    def __init__(self, layout: BatchLayout, dtables: DeviceTables, device):
+        self.L = layout
+        self.dt = dtables
+        self.device = torch.device(device)
+        self.lib = _lib.load()
+        dev = self.device
+        self.d = {}
+        for k, a in layout.arrays.items():
+            self.d[k] = torch.as_tensor(np.ascontiguousarray(a), device=dev)
+        z = lambda k: torch.empty(k, dtype=torch.float64, device=dev)
+        L = layout
+        nm, nv = L.nmat_total, L.nvec_total
+        natt = int(L.nat.sum())
+        self.S, self.H0 = z(nm), z(nm)
+        self.F, self.C, self.T, self.P = z(nm), z(nm), z(nm), z(nm)
+        self.gamma = z(max(L.ngam_total, 1))
+        self.cn = z(natt)
+        self.pot, self.vnew, self.q, self.eorb, self.emo = z(nv), z(nv), z(nv), z(nv), z(nv)
+        self.occ = z(2 * nv)
+        self.qsh = z(int(L.sh0[-1]))
+        self.out = torch.zeros(L.B * _lib.OUT_STRIDE, dtype=torch.float64, device=dev)
+        self.grad = z(3 * natt)
+        self.dedcn = z(natt)
+        self.hist_x = self.hist_f = None
+        order = np.argsort(-L.n, kind="stable").astype(np.int32)  # largest first
+        self.order = torch.as_tensor(order, device=dev)
+        a = self.d
+        self.batch = _lib.XtbBatch(
+            nmol=L.B, nat_total=natt, nsh_total=int(L.sh0[-1]), nvec_total=nv,
+            max_n=int(L.n.max()), max_nsh=int(L.nsh.max()), max_nat=int(L.nat.max()),
+            ntile_sh=L.ntile_sh, ntile_at=L.ntile_at, pad0=0, nmat_total=nm,
+            ngam_total=L.ngam_total,
+            **{k: a[k].data_ptr() for k in (
+                "mol_nat", "mol_nsh", "mol_nao", "mol_n", "mol_atom0", "mol_sh0", "mol_vec0",
+                "mol_mat0", "mol_gam0", "mol_nel", "atom_sp", "atom_mol", "atom_sh0", "atom_nsh",
+                "sh_atom", "sh_us", "sh_ao", "ao_sh", "n0", "tile_sh", "tile_at")},
+        )
+
+    def _state(self) -> _lib.XtbScfState:
+        return _lib.XtbScfState(
+            F=self.F.data_ptr(), C=self.C.data_ptr(), T=self.T.data_ptr(), P=self.P.data_ptr(),
+            pot=self.pot.data_ptr(), vnew=self.vnew.data_ptr(), q=self.q.data_ptr(),
+            eorb=self.eorb.data_ptr(), emo=self.emo.data_ptr(), occ=self.occ.data_ptr(),
+            hist_x=self.hist_x.data_ptr(), hist_f=self.hist_f.data_ptr(), qsh=self.qsh.data_ptr(),
+            out=self.out.data_ptr(), order=self.order.data_ptr(),
+        )
+
+    def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None) -> None:
+        """positions: (nat_total, 3) float64 on the device, bohr."""
+        if positions.dtype != torch.float64 or not positions.is_contiguous():
+            raise ValueError("positions must be contiguous float64")
+        if positions.shape != (int(self.L.nat.sum()), 3):
+            raise ValueError("positions do not match the batch layout")
+        self.positions = positions
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        s = ct.c_void_p(st.cuda_stream)
+        lib, b, p = self.lib, ct.byref(self.batch), ct.byref(self.dt.s)
+        g = opts.generations
+        if self.hist_x is None or self.hist_x.numel() != (g + 1) * self.L.nvec_total:
+            self.hist_x = torch.empty((g + 1) * self.L.nvec_total, dtype=torch.float64, device=self.device)
+            self.hist_f = torch.empty_like(self.hist_x)
+        pos = positions.data_ptr()
+        _lib.check(lib.xtb_build_s_h0(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
+                                      self.cn.data_ptr(), s), "xtb_build_s_h0")
+        _lib.check(lib.xtb_build_gamma(b, p, pos, self.gamma.data_ptr(), s), "xtb_build_gamma")
+        _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
+        o = opts.to_struct(want_w=forces, padded=self.L.padded)
+        self.state = self._state()
+        _lib.check(lib.xtb_scf(b, p, self.S.data_ptr(), self.H0.data_ptr(),
+                               self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state), s),
+                   "xtb_scf")
+        if forces:
+            self.run_forces(stream)
+
+    def run_forces(self, stream=None) -> None:
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        s = ct.c_void_p(st.cuda_stream)
+        _lib.check(self.lib.xtb_forces(ct.byref(self.batch), ct.byref(self.dt.s),
+                                       self.positions.data_ptr(), self.cn.data_ptr(),
+                                       self.gamma.data_ptr(), ct.byref(self.state),
+                                       self.dedcn.data_ptr(), self.grad.data_ptr(), s),
+                   "xtb_forces")
+
+    # ---- result views -------------------------------------------------------
+    def records(self) -> torch.Tensor:
+        return self.out.view(self.L.B, _lib.OUT_STRIDE)
+
+    def energies(self) -> torch.Tensor:
+        """Total energy per molecule: E_scf + (-TS) + E_rep."""
+        r = self.records()
+        return r[:, _lib.OUT_ESCF] + r[:, _lib.OUT_FENERGY] + r[:, _lib.OUT_EREP]
+
+    def gradient(self) -> torch.Tensor:
+        return self.grad.view(-1, 3)
+
+    def orbital_charges(self, b: int) -> torch.Tensor:
+        v0 = int(self.L.vec0[b])
+        return self.q[v0: v0 + int(self.L.nao[b])]
+
+    def matrix(self, buf: torch.Tensor, b: int) -> torch.Tensor:
+        n = int(self.L.n[b])
+        m0 = int(self.L.mat0[b])
+        return buf[m0: m0 + n * n].view(n, n)

@@ -1,0 +1,234 @@
+/*
+ * xtb_hip.h -- C ABI of the MI355X (gfx950) GFN1-xTB single-point library.
+ *
+ * The library computes, for a batch of molecules stored in a ragged layout,
+ * the quantities upstream dxtb computes on its GFN1 single-point path:
+ *
+ *   xtb_build_s_h0   overlap S, core Hamiltonian H0, coordination numbers
+ *                    replaces Integrals.build_overlap / build_hcore
+ *                    (src/dxtb/_src/integral/container.py:178-246,
+ *                     src/dxtb/_src/xtb/base.py:251-360) and cn_d3
+ *                    (called at src/dxtb/_src/xtb/gfn1.py:57-64)
+ *   xtb_build_gamma  shell-resolved ES2 Coulomb matrix and ES3 hardness
+ *                    derivatives; replaces ES2.get_cache / ES3.get_cache
+ *                    (src/dxtb/_src/components/interactions/coulomb/
+ *                     secondorder.py:216-276,799-870; thirdorder.py:182-250)
+ *   xtb_scf          persistent per-molecule SCF (Fock build, generalized
+ *                    eigensolve, Fermi smearing, density, Mulliken charges,
+ *                    ES2/ES3 potential, Anderson mixing, energies); replaces
+ *                    scf.solve -> SelfConsistentFieldFull
+ *                    (src/dxtb/_src/scf/iterator.py:51-144,
+ *                     src/dxtb/_src/scf/unrolling/default.py:71-401)
+ *   xtb_repulsion    classical repulsion energy per molecule; replaces
+ *                    Repulsion.get_energy
+ *                    (src/dxtb/_src/components/classicals/repulsion/
+ *                     base.py:269-334, rep.py:65-98)
+ *   xtb_forces       analytic nuclear gradient dE/dR; replaces
+ *                    AnalyticalCalculator.forces_analytical
+ *                    (src/dxtb/_src/calculators/types/analytical.py:61-222)
+ *                    with GFN1Hamiltonian.get_gradient
+ *                    (src/dxtb/_src/xtb/gfn1.py:185-408)
+ *
+ * Conventions
+ *  - All array pointers are DEVICE pointers owned by the caller; the library
+ *    allocates no device memory.  Sizes of the work buffers follow from the
+ *    batch descriptor (see xtb_workspace_doubles).
+ *  - Every call is stream ordered on `stream` (a hipStream_t passed as
+ *    void*), returns an int32 status (XTB_OK or an XTB_E* code) and never
+ *    throws.  Per-molecule outcomes (SCF convergence, Fermi failure,
+ *    non-finite values) are written to device memory (xtb_scf_state.out).
+ *  - Units: bohr, Hartree.  Arithmetic is IEEE FP64 throughout.
+ *  - Matrices are row-major, one n x n block per molecule at offset
+ *    mol_mat0[b] (in doubles) of a packed buffer; n = mol_n[b] is the
+ *    matrix dimension (the molecule's own orbital count in the ragged
+ *    layout, the batch maximum in the padded layout), of which the first
+ *    mol_nao[b] orbitals are real.
+ */
+#ifndef XTB_HIP_H
+#define XTB_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XTB_ABI_VERSION 1
+
+/* call status */
+#define XTB_OK 0
+#define XTB_EINVAL 1      /* bad argument or descriptor */
+#define XTB_EHIP 2        /* a HIP runtime call failed */
+#define XTB_ETOOBIG 3     /* a size exceeds a kernel limit */
+
+/* per-molecule status word (xtb_scf_state.out[b*XTB_OUT_STRIDE+XTB_OUT_STATUS]) */
+#define XTB_MOL_CONVERGED 0
+#define XTB_MOL_MAXITER 1   /* SCF not converged within maxiter */
+#define XTB_MOL_FERMI 2     /* Fermi level failed to converge */
+#define XTB_MOL_NONFINITE 4 /* NaN/Inf met */
+#define XTB_MOL_JACOBI 8    /* eigensolver hit its sweep limit */
+
+/* layout of the per-molecule result record */
+#define XTB_OUT_STRIDE 16
+#define XTB_OUT_ESCF 0      /* E_h0 + E_es2 + E_es3 */
+#define XTB_OUT_EH0 1
+#define XTB_OUT_EES2 2
+#define XTB_OUT_EES3 3
+#define XTB_OUT_FENERGY 4   /* electronic free energy -TS */
+#define XTB_OUT_ITER 5      /* SCF iterations (as counted upstream) */
+#define XTB_OUT_STATUS 6
+#define XTB_OUT_EFERMI_A 7
+#define XTB_OUT_EFERMI_B 8
+#define XTB_OUT_EREP 9      /* written by xtb_repulsion */
+#define XTB_OUT_SWEEPS 10   /* total Jacobi sweeps spent */
+
+/* Ragged batch descriptor.  All pointers are device pointers. */
+typedef struct xtb_batch {
+    int32_t nmol;
+    int32_t nat_total;   /* atoms over the batch */
+    int32_t nsh_total;   /* shells over the batch */
+    int32_t nvec_total;  /* sum of mol_n */
+    int32_t max_n;       /* max of mol_n */
+    int32_t max_nsh;     /* max of mol_nsh */
+    int32_t max_nat;     /* max of mol_nat */
+    int32_t ntile_sh;    /* entries of the shell-pair tile list */
+    int32_t ntile_at;    /* entries of the atom-pair tile list */
+    int32_t pad0;
+    int64_t nmat_total;  /* doubles of one packed n x n buffer */
+    int64_t ngam_total;  /* doubles of the packed nsh x nsh buffer */
+    /* per molecule [nmol] */
+    const int32_t* mol_nat;
+    const int32_t* mol_nsh;
+    const int32_t* mol_nao;
+    const int32_t* mol_n;
+    const int32_t* mol_atom0;
+    const int32_t* mol_sh0;
+    const int32_t* mol_vec0;
+    const int64_t* mol_mat0;
+    const int64_t* mol_gam0;
+    const double* mol_nel;   /* [2*nmol] alpha, beta electrons */
+    /* per atom [nat_total] */
+    const int32_t* atom_sp;  /* species index into xtb_params */
+    const int32_t* atom_mol;
+    const int32_t* atom_sh0; /* first shell of the atom, local to the molecule */
+    const int32_t* atom_nsh; /* shells on the atom */
+    /* per shell [nsh_total] */
+    const int32_t* sh_atom;  /* atom index local to the molecule */
+    const int32_t* sh_us;    /* unique-shell index into xtb_params */
+    const int32_t* sh_ao;    /* first orbital, local to the molecule */
+    /* per orbital slot [nvec_total] */
+    const int32_t* ao_sh;    /* shell local to the molecule, -1 = padding */
+    const double* n0;        /* reference occupation */
+    /* 16x16 tiles of the shell-pair and atom-pair spaces (upper triangle) */
+    const int32_t* tile_sh;  /* [3*ntile_sh]: molecule, i0, j0 */
+    const int32_t* tile_at;  /* [3*ntile_at]: molecule, a0, b0 */
+} xtb_batch;
+
+/* GFN1 parameters, reduced to the species/shells present in the batch. */
+typedef struct xtb_params {
+    int32_t nsp;             /* species */
+    int32_t nus;             /* unique shells */
+    const double* sp_rad;    /* [nsp] atomic radius of the H0 distance polynomial, bohr */
+    const double* sp_rcov;   /* [nsp] D3 covalent radius, bohr */
+    const double* sp_gam3;   /* [nsp] third-order Hubbard derivative */
+    const double* sp_zeff;   /* [nsp] repulsion effective charge */
+    const double* sp_arep;   /* [nsp] repulsion exponent */
+    const double* us_level;  /* [nus] self-energy, Hartree */
+    const double* us_kcn;    /* [nus] CN shift of the self-energy, Hartree */
+    const double* us_shpoly; /* [nus] distance-polynomial coefficient */
+    const double* us_hard;   /* [nus] shell hardness lgam*gam */
+    const int32_t* us_l;     /* [nus] angular momentum (0 or 1) */
+    const int32_t* us_nprim; /* [nus] primitives (<= 8) */
+    const double* us_alpha;  /* [nus*8] primitive exponents */
+    const double* us_coeff;  /* [nus*8] normalised contraction coefficients */
+    const double* us_K;      /* [nus*nus] K_ab = hscale*kpair*X(EN) or hscale */
+    double kexp;             /* repulsion distance exponent */
+    double rep_cutoff;       /* bohr */
+    double cn_cutoff;        /* bohr */
+    double cn_steep;         /* steepness of the counting function */
+    double int_cutoff;       /* bohr, overlap cutoff */
+} xtb_params;
+
+typedef struct xtb_scf_opts {
+    int32_t maxiter;
+    int32_t mix_guess;
+    int32_t generations;     /* Anderson history length */
+    int32_t fermi_maxiter;
+    int32_t want_w;          /* also form the energy-weighted density */
+    int32_t jacobi_max_sweeps;
+    int32_t padded;          /* 1: padded layout semantics */
+    int32_t pad0;
+    double x_atol;
+    double x_atol_max;
+    double kt;               /* electronic temperature, Hartree */
+    double damp;
+    double damp_init;
+    double diag_offset;
+    double fermi_thresh;
+    double jacobi_tol;
+} xtb_scf_opts;
+
+/* Work and result buffers of xtb_scf (device, caller-owned). */
+typedef struct xtb_scf_state {
+    double* F;      /* packed n x n: Fock / transformed Fock */
+    double* C;      /* packed n x n: MO coefficients stored by vector,
+                       C[k*n + mu] = coefficient of orbital mu in MO k */
+    double* T;      /* packed n x n: scratch; energy-weighted density W on exit if want_w */
+    double* P;      /* packed n x n: density */
+    double* pot;    /* [nvec] orbital potential of the final charges */
+    double* vnew;   /* [nvec] scratch: potential produced by an SCF step */
+    double* q;      /* [nvec] orbital Mulliken charges */
+    double* eorb;   /* [nvec] diag(P H0) */
+    double* emo;    /* [nvec] orbital energies, ascending */
+    double* occ;    /* [2*nvec] alpha/beta occupations, ascending-energy order */
+    double* hist_x; /* [(generations+1)*nvec] */
+    double* hist_f; /* [(generations+1)*nvec] */
+    double* qsh;    /* [nsh_total] shell charges of the final density */
+    double* out;    /* [nmol*XTB_OUT_STRIDE] result records */
+    const int32_t* order; /* [nmol] processing order (largest first) */
+} xtb_scf_state;
+
+int32_t xtb_abi_version(void);
+
+/* sizeof(xtb_batch), sizeof(xtb_params), sizeof(xtb_scf_opts),
+ * sizeof(xtb_scf_state): lets bindings verify their struct mirrors. */
+int32_t xtb_abi_struct_sizes(int64_t* out4);
+
+/* Doubles each buffer class needs for this batch:
+ * out[0] = packed n x n buffer, out[1] = packed nsh x nsh, out[2] = nvec,
+ * out[3] = history (generations+1)*nvec.  Host-only, no device access. */
+int32_t xtb_workspace_doubles(const xtb_batch* batch, int32_t generations,
+                              int64_t* out4);
+
+int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
+                       const double* positions, double* S, double* H0,
+                       double* cn, void* stream);
+
+int32_t xtb_build_gamma(const xtb_batch* batch, const xtb_params* par,
+                        const double* positions, double* gamma, void* stream);
+
+/* Dynamic LDS the SCF kernel will request for this batch, and the part of
+ * it available to the eigensolver (doubles).  Host-only. */
+int32_t xtb_scf_lds_bytes(const xtb_batch* batch, int32_t* bytes, int32_t* big_cap);
+
+int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par,
+                const double* S, const double* H0, const double* gamma,
+                const xtb_scf_opts* opts, const xtb_scf_state* state,
+                void* stream);
+
+int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
+                      const double* positions, double* out, void* stream);
+
+/* Nuclear gradient dE/dR of E = E_scf + (-TS) + E_rep, per atom [3*nat_total].
+ * Needs the converged P, W (state.T when want_w), the orbital potential,
+ * shell charges and CN from the same batch.  `dedcn` is [nat_total] scratch. */
+int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
+                   const double* positions, const double* cn,
+                   const double* gamma, const xtb_scf_state* state,
+                   double* dedcn, double* grad, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XTB_HIP_H */

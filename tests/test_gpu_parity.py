@@ -1,0 +1,88 @@
+"""HIP path vs the CPU oracle on the same seeded inputs (FP64).
+
+Tolerances: integrals 1e-12 (absolute, O(1) entries), energies 1e-9 Eh,
+orbital Mulliken charges 1e-8 e, forces 1e-8 Eh/bohr -- tighter than the
+1e-8 Eh / 1e-7 e / 1e-7 Eh/bohr parity targets.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.gfn1 as og
+from conftest import mol
+from dxtb_amd import workloads as W
+from helpers import TIGHT_ORACLE, block, run_native
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _mols(fx):
+    out = [mol(fx, k) for k in ("h2o", "ch4", "lys_xao")]
+    out.append(W.config0())
+    return out
+
+
+def test_integrals_vs_oracle(fx):
+    mols = _mols(fx)
+    print("case: integrals h2o ch4 lys_xao c8h18", flush=True)
+    pipe = run_native(mols, forces=False)
+    for b, (z, p) in enumerate(mols):
+        pos = torch.tensor(p)
+        bas = og.build_basis(z)
+        S = og.overlap(bas, pos).numpy()
+        cn = og.coordination_number(list(z), pos).numpy()
+        H = og.core_hamiltonian(bas, pos, torch.tensor(S), torch.tensor(cn)).numpy()
+        G = og.gamma_matrix(bas, pos).numpy()
+        a0 = int(pipe.L.atom0[b])
+        assert np.abs(block(pipe, pipe.S, b) - S).max() < 1e-13
+        assert np.abs(block(pipe, pipe.H0, b) - H).max() < 1e-12
+        assert np.abs(pipe.cn[a0:a0 + len(z)].cpu().numpy() - cn).max() < 1e-12
+        g0, m = int(pipe.L.gam0[b]), int(pipe.L.nsh[b])
+        Gn = pipe.gamma[g0:g0 + m * m].view(m, m).cpu().numpy()
+        assert np.abs(Gn - G).max() < 1e-13
+        erep = og.repulsion(list(z), pos).sum().item()
+        assert abs(pipe.records()[b, 9].item() - erep) < 1e-12
+
+
+def test_energy_charges_forces_vs_oracle(fx):
+    mols = _mols(fx)
+    print("case: scf+forces h2o ch4 lys_xao c8h18", flush=True)
+    pipe = run_native(mols, forces=True)
+    rec = pipe.records().cpu().numpy()
+    E = pipe.energies().cpu().numpy()
+    G = pipe.gradient().cpu().numpy()
+    for b, (z, p) in enumerate(mols):
+        r = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
+        assert int(rec[b, 6]) == 0, f"status {rec[b, 6]}"
+        assert abs(E[b] - r["energy"]) < 1e-9
+        assert abs(rec[b, 0] - r["scf"]) < 1e-9
+        q = pipe.orbital_charges(b).cpu().numpy()
+        assert np.abs(q - r["charges_orbital"].numpy()).max() < 1e-8
+        a0 = int(pipe.L.atom0[b])
+        assert np.abs(G[a0:a0 + len(z)] - r["gradient"].numpy()).max() < 1e-8
+
+
+def test_scf_energy_vs_tblite(fx):
+    keys = ("h2", "h2o", "ch4", "lys_xao", "c60")
+    mols = [mol(fx, k) for k in keys]
+    print("case: tblite energies", flush=True)
+    pipe = run_native(mols)
+    rec = pipe.records().cpu().numpy()
+    for b, k in enumerate(keys):
+        assert abs(rec[b, 0] - float(fx[f"{k}_escf"])) < 1e-9, k
+
+
+def test_padded_layout_matches_ragged(fx):
+    mols = _mols(fx)
+    print("case: padded vs ragged", flush=True)
+    a = run_native(mols, padded=False, forces=True)
+    b = run_native(mols, padded=True, forces=True)
+    assert np.abs(a.energies().cpu().numpy() - b.energies().cpu().numpy()).max() < 1e-10
+    assert np.abs(a.gradient().cpu().numpy() - b.gradient().cpu().numpy()).max() < 1e-10

@@ -1,0 +1,90 @@
+"""Host-side logic: parameter tables, batch layout, C-ABI library surface."""
+
+import ctypes as ct
+
+import numpy as np
+import pytest
+
+import dxtb_amd._lib as L
+from dxtb_amd import workloads as W
+from dxtb_amd.engine import BatchLayout, alpha_beta
+from dxtb_amd.param import SpeciesTables, slater_to_gauss
+
+import oracle.gfn1 as og
+
+
+def test_library_loads_and_exports_every_symbol():
+    lib = L.load()
+    for name in L.EXPORTED:
+        assert hasattr(lib, name)
+    # every function declared in include/xtb_hip.h is exported
+    import re
+    from pathlib import Path
+    hdr = (Path(__file__).resolve().parent.parent / "include" / "xtb_hip.h").read_text()
+    declared = set(re.findall(r"^int32_t (xtb_\w+)\(", hdr, flags=re.M))
+    assert declared == set(L.EXPORTED)
+    sizes = (ct.c_int64 * 4)()
+    assert lib.xtb_abi_struct_sizes(sizes) == 0
+    assert list(sizes) == [ct.sizeof(s) for s in (L.XtbBatch, L.XtbParams, L.XtbScfOpts, L.XtbScfState)]
+
+
+def test_species_tables_match_oracle_basis():
+    tb = SpeciesTables([1, 6, 7, 8])
+    for z in (1, 6, 7, 8):
+        bas = og.build_basis([z])
+        s = tb.sp_of_z[z]
+        base = tb.us_base[s]
+        for k in range(tb.nsh_of_sp[s]):
+            n = tb.us_nprim[base + k]
+            a = tb.us_alpha[(base + k) * 8:(base + k) * 8 + n]
+            c = tb.us_coeff[(base + k) * 8:(base + k) * 8 + n]
+            assert np.allclose(a, bas.shell_alpha[k].numpy(), rtol=1e-15, atol=0)
+            assert np.allclose(c, bas.shell_coeff[k].numpy(), rtol=1e-13, atol=0)
+    K = og._shell_pair_k(og.build_basis([1, 6, 7, 8]), og.load_params()).numpy()
+    assert np.allclose(tb.us_K, K, rtol=1e-15, atol=0)
+
+
+def test_batch_layout_ragged_and_padded():
+    mols = W.config1(count=5)
+    zs = [z for z, _ in mols]
+    tb = SpeciesTables(np.unique(np.concatenate(zs)))
+    for padded in (False, True):
+        lay = BatchLayout(zs, tb, padded=padded, chrg=np.zeros(5))
+        a = lay.arrays
+        for b, z in enumerate(zs):
+            bas = og.build_basis(z)
+            assert lay.nao[b] == bas.nao and lay.nsh[b] == bas.nsh
+            v0 = lay.vec0[b]
+            assert list(a["ao_sh"][v0:v0 + bas.nao]) == bas.ao_shell
+            assert np.all(a["ao_sh"][v0 + bas.nao:lay.vec0[b + 1]] == -1)
+            assert lay.n[b] == (lay.nao.max() if padded else bas.nao)
+            s0 = lay.sh0[b]
+            assert list(a["sh_atom"][s0:s0 + bas.nsh]) == bas.shell_atom
+            nel = float(og._n0(bas).sum())
+            assert a["mol_nel"][2 * b] + a["mol_nel"][2 * b + 1] == pytest.approx(nel)
+        # tiles cover every upper-triangle shell pair exactly once
+        t = a["tile_sh"].reshape(-1, 3)
+        for b in range(5):
+            m = lay.nsh[b]
+            cov = np.zeros((m, m), dtype=int)
+            for mm, i0, j0 in t[t[:, 0] == b]:
+                for i in range(i0, min(i0 + 16, m)):
+                    for j in range(j0, min(j0 + 16, m)):
+                        if j >= i:
+                            cov[i, j] += 1
+            assert np.array_equal(np.triu(cov), np.triu(np.ones((m, m), dtype=int)))
+        assert np.all(np.diff(lay.mat0) % 32 == 0)
+
+
+def test_alpha_beta_split():
+    na, nb = alpha_beta(np.array([10.0, 9.0, 9.0]), None)
+    assert list(na) == [5.0, 5.0, 5.0] and list(nb) == [5.0, 4.0, 4.0]
+    na, nb = alpha_beta(np.array([10.0]), 2)
+    assert (na[0], nb[0]) == (6.0, 4.0)
+
+
+def test_sto_expansion_identical_to_oracle():
+    for (ng, n, l) in [(6, 2, 0), (6, 2, 1), (4, 1, 0), (3, 2, 0)]:
+        a, c = slater_to_gauss(ng, n, l, 1.7)
+        ao, co = og.slater_to_gauss(ng, n, l, 1.7)
+        assert np.allclose(a, ao.numpy(), rtol=1e-15, atol=0) and np.allclose(c, co.numpy(), rtol=1e-15, atol=0)

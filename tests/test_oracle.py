@@ -1,0 +1,140 @@
+"""The CPU oracle against upstream's golden vectors and against mathematics.
+
+Each check is chosen so a plausible mistake fails it: a wrong primitive
+normalisation or p-orbital order fails the tblite pair values and the
+overlap fixture; a wrong radius, CN or unit constant fails the H0 fixture or
+the tblite SCF energies (which discriminate CODATA 2010/2018 Hartree-eV at
+1e-8 relative); a sign or factor slip in the gradient fails both the tblite
+SCF gradients and central finite differences.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import oracle.gfn1 as og
+from conftest import mol
+
+
+def test_pair_overlaps_match_tblite(fx):
+    for k in ("1s1s", "2s1s", "3s4s", "2p1s", "2p2p"):
+        (ngi, ni, li), (ngj, nj, lj) = fx[f"pair_{k}_cgto"]
+        ai, ci = og.slater_to_gauss(int(ngi), int(ni), int(li), 1.0)
+        aj, cj = og.slater_to_gauss(int(ngj), int(nj), int(lj), 1.0)
+        # upstream's vec enters md as (R_j - R_i); our rvec = R_A - R_B
+        rvec = -torch.tensor(fx[f"pair_{k}_vec"])
+        s, ds = og.shell_pair_overlap(int(li), int(lj), ai, ci, aj, cj, rvec, grad=True)
+        assert np.allclose(s.numpy(), fx[f"pair_{k}_s"], atol=1e-5), k
+        # fixture rows: flattened (mi, mj) pairs; columns: derivative with
+        # respect to the x, y, z components of R_A - R_B
+        g = ds.permute(1, 2, 0).reshape(-1, 3).numpy()
+        assert np.allclose(g, fx[f"pair_{k}_g"], atol=1e-5), k
+
+
+@pytest.mark.parametrize("key", ["h2", "h2o", "ch4", "lys_xao"])
+def test_overlap_and_h0_fixtures(fx, key):
+    z, p = mol(fx, key)
+    bas = og.build_basis(z)
+    pos = torch.tensor(p)
+    S = og.overlap(bas, pos)
+    cn = og.coordination_number(list(z), pos)
+    H = og.core_hamiltonian(bas, pos, S, cn)
+    # fixtures are float32: 2e-8 relative on O(1) entries
+    assert np.abs(S.numpy() - fx[f"{key}_overlap"]).max() < 5e-8
+    assert np.abs(H.numpy() - fx[f"{key}_h0"]).max() < 1e-7
+
+
+def test_normalisation_and_orthogonality():
+    bas = og.build_basis([1, 6])
+    pos = torch.tensor([[0.0, 0.0, 0.0], [0.0, 0.0, 40.0]])
+    # the STO-nG tables are normalised to ~1e-9 (as upstream uses them)
+    for a, c, l in zip(bas.shell_alpha, bas.shell_coeff, bas.shell_l):
+        s = og.shell_pair_overlap(l, l, a, c, a, c, torch.zeros(3))
+        assert (torch.diagonal(s) - 1.0).abs().max() < 1e-8
+    # H 2s is explicitly renormalised after Gram-Schmidt: exact to rounding
+    s22 = og.shell_pair_overlap(0, 0, bas.shell_alpha[1], bas.shell_coeff[1],
+                                bas.shell_alpha[1], bas.shell_coeff[1], torch.zeros(3))
+    assert abs(float(s22) - 1.0) < 1e-14
+    s12 = og.shell_pair_overlap(0, 0, bas.shell_alpha[0], bas.shell_coeff[0],
+                                bas.shell_alpha[1], bas.shell_coeff[1], torch.zeros(3))
+    assert abs(float(s12)) < 1e-8
+
+
+def test_overlap_gradient_finite_difference(fx):
+    z, p = mol(fx, "h2o")
+    bas = og.build_basis(z)
+    pos = torch.tensor(p)
+    dS = og.overlap_grad(bas, pos)
+    ao_atom = [bas.shell_atom[s] for s in bas.ao_shell]
+    h = 1e-5
+    for a in range(len(z)):
+        for x in range(3):
+            pp, pm = pos.clone(), pos.clone()
+            pp[a, x] += h
+            pm[a, x] -= h
+            fd = (og.overlap(bas, pp) - og.overlap(bas, pm)) / (2 * h)
+            for i in range(bas.nao):
+                if ao_atom[i] != a:
+                    continue
+                for j in range(bas.nao):
+                    if ao_atom[j] == a:
+                        continue
+                    assert abs(float(fd[i, j] - dS[i, j, x])) < 1e-8
+
+
+TIGHT = {"x_atol": 1e-11, "x_atol_max": 1e-11, "maxiter": 300}
+
+
+@pytest.mark.parametrize("key", ["h2", "h2o", "ch4", "lys_xao"])
+def test_scf_energy_matches_tblite(fx, key):
+    z, p = mol(fx, key)
+    r = og.singlepoint(z, p, opts=TIGHT)
+    assert r["converged"]
+    assert abs(r["scf"] - float(fx[f"{key}_escf"])) < 1e-9
+
+
+@pytest.mark.parametrize("key", ["h2", "h2o", "lys_xao"])
+def test_repulsion_matches_tblite(fx, key):
+    z, p = mol(fx, key)
+    e = og.repulsion(list(z), torch.tensor(p))
+    assert abs(float(e.sum()) - float(fx[f"{key}_erep"])) < 1e-12
+    if key == "h2o":
+        _, g = og.repulsion(list(z), torch.tensor(p), grad=True)
+        assert np.abs(g.numpy() - fx["h2o_grep"]).max() < 1e-12
+
+
+@pytest.mark.parametrize("key", ["h2", "h2o", "ch4"])
+def test_scf_gradient_matches_tblite(fx, key):
+    z, p = mol(fx, key)
+    r = og.singlepoint(z, p, opts=TIGHT, with_forces=True)
+    _, grep = og.repulsion(list(z), torch.tensor(p), grad=True)
+    g = (r["gradient"] - grep).numpy()
+    # tblite gradients are float32 and SCF-converged loosely: 1e-6
+    assert np.abs(g - fx[f"{key}_gscf"]).max() < 1e-6
+
+
+def test_forces_central_difference(fx):
+    z, p = mol(fx, "h2o")
+    r = og.singlepoint(z, p, opts=TIGHT, with_forces=True)
+    h = 1e-4
+    for a in range(len(z)):
+        for x in range(3):
+            pp, pm = p.copy(), p.copy()
+            pp[a, x] += h
+            pm[a, x] -= h
+            fd = (og.singlepoint(z, pp, opts=TIGHT)["energy"]
+                  - og.singlepoint(z, pm, opts=TIGHT)["energy"]) / (2 * h)
+            assert abs(fd - float(r["gradient"][a, x])) < 1e-8
+
+
+def test_fermi_closed_form():
+    # two-level system with a tiny gap: occupations follow 1/(1+exp) exactly
+    kt = 300 * og.KELVIN2AU
+    e = torch.tensor([-0.5, -0.1, -0.1 + 1e-4, 0.3], dtype=torch.float64)
+    f = og.fermi_occupation(2.0, e, kt)
+    assert abs(float(f.sum()) - 2.0) < 1e-8
+    assert float(f[0]) == pytest.approx(1.0, abs=1e-15)
+    ef = -0.1 + 0.5e-4  # symmetric pair around the midpoint
+    assert float(f[1]) == pytest.approx(1.0 / (1.0 + math.exp((-0.1 - ef) / kt)), rel=1e-6)
