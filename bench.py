@@ -1,0 +1,176 @@
+"""Benchmark: GFN1 energy+forces molecules/s of the ragged serving loop.
+
+Workload: the serving stream of log-uniform 5..200-atom molecules (70 % CHNO
+chains, 30 % water clusters), energy + analytic forces in FP64, seed 4.  A
+"step" serves the next `--chunk` requests of the stream with all of them
+available at the start (saturation throughput).  The JSON line also carries
+
+* the plain padded-batch drop-in on the same requests (arrival order,
+  `--padded-batch` molecules per padded batch) and the ragged/padded ratio;
+* p50/p99 latency of the serving loop under Poisson arrivals at 70 % of the
+  measured saturation rate;
+* a roofline entry for the dominant kernel (the persistent SCF kernel,
+  FP64): algorithmic flops per launch / mean launch duration from HIP
+  events, in a pass where launches do not overlap;
+* the CPU baseline: the oracle (a PyTorch FP64 restatement of the upstream
+  algorithm) timed on this host on a bounded prefix of the same stream.
+
+    python bench.py [--steps K] [--warmup W] [--chunk R]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+METRIC = "GFN1 energy+forces molecules/s vs plain padded batch; p50/p99 latency @1e-8 Eh"
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _requests(stream, start, count):
+    from dxtb_amd.serving import Request
+
+    out = []
+    for i, (t, z, p) in enumerate(stream[start:start + count]):
+        out.append(Request(rid=start + i, numbers=z, positions=p, arrival=t))
+    return out
+
+
+def cpu_baseline(stream, seconds: float) -> dict:
+    import torch
+
+    import oracle.gfn1 as og  # CPU restatement of the upstream algorithm
+
+    opts = {"x_atol": 1e-10, "x_atol_max": 1e-10, "maxiter": 300}
+    t0 = time.perf_counter()
+    n = 0
+    atoms = 0
+    for _, z, p in stream:
+        og.singlepoint(z, p, opts=opts, with_forces=True)
+        n += 1
+        atoms += len(z)
+        if time.perf_counter() - t0 > seconds:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "molecules/s", "cores": torch.get_num_threads(),
+            "kind": "port",
+            "sample": f"first {n} requests of the same stream ({atoms} atoms), energy+forces, "
+                      f"sequential, {dt:.1f} s, torch {torch.__version__} CPU FP64, "
+                      f"{os.cpu_count()} logical CPUs"}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--chunk", type=int, default=600)
+    ap.add_argument("--latency-requests", type=int, default=600)
+    ap.add_argument("--padded-batch", type=int, default=64)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-padded", action="store_true")
+    ap.add_argument("--quiet", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
+              "(molecules are independent: shard requests across GPUs by a router).",
+              file=sys.stderr)
+        return 2
+
+    import torch
+
+    from dxtb_amd import workloads as W
+    from dxtb_amd.engine import ScfOptions
+    from dxtb_amd.serving import RaggedServer, padded_dropin
+
+    log = (lambda *x: None) if a.quiet else (lambda *x: print(*x, file=sys.stderr, flush=True))
+    dev = torch.device("cuda:0")
+    opts = ScfOptions(x_atol=1e-10, x_atol_max=1e-10, maxiter=300)
+    total = (a.warmup + a.steps) * a.chunk + a.latency_requests
+    stream = W.config4(seed=4, count=total)
+    log(f"stream: {total} requests generated")
+
+    server = RaggedServer(dev, opts=opts)
+    pos = 0
+    for w in range(a.warmup):
+        server.run_stream(_requests(stream, pos, a.chunk), realtime=False)
+        pos += a.chunk
+        log(f"warmup {w} done")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    bad = 0
+    for k in range(a.steps):
+        reqs = _requests(stream, pos, a.chunk)
+        server.run_stream(reqs, realtime=False)
+        bad += sum(1 for r in reqs if server.results[r.rid].status != 0)
+        pos += a.chunk
+        log(f"step {k} done")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    value = a.steps * a.chunk / (t1 - t0)
+    ms_step = (t1 - t0) * 1e3 / a.steps
+    log(f"ragged: {value:.1f} molecules/s")
+
+    # roofline of the SCF kernel: one stream, launches do not overlap
+    rs = RaggedServer(dev, opts=opts, streams=1, timing=True)
+    rs.run_stream(_requests(stream, a.warmup * a.chunk, a.chunk), realtime=False)
+    ms = np.array([x[0] for x in rs.kernel_log])
+    fl = np.array([x[1] for x in rs.kernel_log])
+    achieved = float(fl.mean() / (ms.mean() * 1e-3) / 1e12)
+    log(f"scf kernel: {len(ms)} launches, mean {ms.mean():.2f} ms, {achieved:.3f} TFLOP/s")
+
+    # latency under Poisson arrivals at 70 % of the saturation rate
+    rate = 0.7 * value
+    lat_reqs = _requests(stream, (a.warmup + a.steps) * a.chunk, a.latency_requests)
+    rng = np.random.default_rng(44)
+    t = 0.0
+    for r in lat_reqs:
+        t += rng.exponential(1.0 / rate)
+        r.arrival = t
+    ls = RaggedServer(dev, opts=opts)
+    lat = ls.run_stream(lat_reqs, realtime=True)
+    log(f"latency @ {rate:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
+
+    padded = None
+    if not a.no_padded:
+        pr = _requests(stream, a.warmup * a.chunk, a.chunk)
+        padded = padded_dropin(pr, a.padded_batch, dev, opts=opts)["molecules_per_s"]
+        log(f"padded drop-in: {padded:.1f} molecules/s")
+
+    cpu = None if a.no_cpu else cpu_baseline(stream, a.cpu_seconds)
+    if cpu:
+        log(f"cpu baseline: {cpu['value']:.3f} molecules/s")
+
+    out = {
+        "metric": METRIC, "value": round(value, 3), "unit": "molecules/s", "n_gpus": 1,
+        "steps": a.steps, "warmup": a.warmup, "ms_per_step": round(ms_step, 2),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "serving stream (log-uniform 5-200 atoms; 70% CHNO chains, 30% water "
+                        "clusters), energy + analytic forces, FP64, SCF x_atol 1e-10",
+            "requests_per_step": a.chunk, "seed": 4,
+            "padded_dropin_molecules_per_s": None if padded is None else round(padded, 3),
+            "ragged_over_padded": None if padded is None else round(value / padded, 3),
+            "padded_batch": a.padded_batch,
+            "latency_rate_per_s": round(rate, 2), "p50_ms": round(lat["p50_ms"], 2),
+            "p99_ms": round(lat["p99_ms"], 2), "nonconverged": bad,
+        },
+        "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
+                     "traffic": None, "kernel": "xtb::k_scf",
+                     "launches": int(len(ms)), "mean_launch_ms": round(float(ms.mean()), 3)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

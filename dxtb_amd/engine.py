@@ -252,8 +252,12 @@ class Pipeline:
             out=self.out.data_ptr(), order=self.order.data_ptr(),
         )
 
-    def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None) -> None:
-        """positions: (nat_total, 3) float64 on the device, bohr."""
+    def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None,
+            timing: bool = False) -> None:
+        """positions: (nat_total, 3) float64 on the device, bohr.
+
+        With ``timing`` HIP events bracket the SCF launch on its stream
+        (``self.scf_events``) for kernel-duration measurements."""
         if positions.dtype != torch.float64 or not positions.is_contiguous():
             raise ValueError("positions must be contiguous float64")
         if positions.shape != (int(self.L.nat.sum()), 3):
@@ -273,9 +277,15 @@ class Pipeline:
         _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
         o = opts.to_struct(want_w=forces, padded=self.L.padded)
         self.state = self._state()
+        if timing:
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record(st)
         _lib.check(lib.xtb_scf(b, p, self.S.data_ptr(), self.H0.data_ptr(),
                                self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state), s),
                    "xtb_scf")
+        if timing:
+            ev1.record(st)
+            self.scf_events = (ev0, ev1)
         if forces:
             self.run_forces(stream)
 
@@ -287,6 +297,18 @@ class Pipeline:
                                        self.gamma.data_ptr(), ct.byref(self.state),
                                        self.dedcn.data_ptr(), self.grad.data_ptr(), s),
                    "xtb_forces")
+
+    def scf_flops(self) -> float:
+        """Algorithmic FP64 flops of the SCF launch (fixed convention): per
+        molecule (iterations + 1) x (9 1/3 n^3 eigen + 2 n^2 k density)
+        + 5 1/3 n^3 orthogonaliser + 2 n^2 k energy-weighted density, with
+        k = occupied orbitals (alpha count) and the measured iterations."""
+        rec = self.records().cpu().numpy()
+        n = self.L.n.astype(np.float64)
+        k = self.L.nel_a.astype(np.float64)
+        it = rec[:, _lib.OUT_ITER] + 1.0
+        return float(np.sum(it * (28.0 / 3.0 * n**3 + 2.0 * n * n * k) + 16.0 / 3.0 * n**3
+                            + 2.0 * n * n * k))
 
     # ---- result views -------------------------------------------------------
     def records(self) -> torch.Tensor:

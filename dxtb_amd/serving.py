@@ -1,0 +1,268 @@
+"""Ragged continuous-batching serving loop for GFN1 energy + forces.
+
+Requests (one molecule each, arbitrary size) arrive over time.  The server
+
+* sorts waiting requests into size classes by orbital count, so that one
+  launch holds molecules of similar cost (a launch finishes when its largest
+  molecule does, and the SCF kernel's LDS plan depends on the largest n);
+* admits a micro-batch from the class whose oldest request has waited
+  longest, as soon as a stream slot is free and the device-memory footprint
+  of everything in flight stays under the HBM budget;
+* launches the whole pipeline (integrals, persistent per-molecule SCF,
+  analytic forces) for the micro-batch on its own HIP stream, so launches
+  overlap on the device: one workgroup per molecule, and workgroups of a
+  newer launch fill compute units that molecules of an older launch have
+  released by converging;
+* keeps each molecule's SCF state (potential, Anderson history, orbitals,
+  density) resident in that launch's device buffers until its results are
+  copied out, then frees the buffers and backfills with waiting requests.
+
+Every request's result is computed by exactly the same kernels whatever its
+neighbours are; ``tests/test_gpu_serving.py`` checks alone == in the mix.
+"""
+
+from __future__ import annotations
+
+import time
+from collections import deque
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
+from .param import SpeciesTables, species_shells
+
+DEFAULT_EDGES = (48, 96, 160, 256, 400, 1024)
+DEFAULT_MAXB = (512, 512, 384, 256, 192, 128)
+
+
+@dataclass
+class Request:
+    rid: int
+    numbers: np.ndarray
+    positions: np.ndarray  # bohr
+    arrival: float = 0.0
+    nao: int = 0
+    energy: float | None = None
+    forces: np.ndarray | None = None
+    done: float | None = None
+    status: int = 0
+    iterations: int = 0
+
+
+@dataclass
+class _Flight:
+    pipe: Pipeline
+    reqs: list
+    stream: torch.cuda.Stream
+    event: torch.cuda.Event
+    bytes: int
+    t_launch: float
+
+
+@dataclass
+class ServerStats:
+    completed: int = 0
+    launches: int = 0
+    admission_stalls: int = 0
+    peak_bytes: int = 0
+    latencies: list = field(default_factory=list)
+
+
+def orbital_count(numbers: np.ndarray) -> int:
+    return int(sum(sum(2 * l + 1 for l, *_ in species_shells(int(z))) for z in numbers))
+
+
+class RaggedServer:
+    def __init__(self, device="cuda:0", opts: ScfOptions | None = None, streams: int = 4,
+                 edges=DEFAULT_EDGES, max_batch=DEFAULT_MAXB, hbm_budget_gb: float = 64.0,
+                 elements=(1, 6, 7, 8), forces: bool = True, max_wait_s: float = 0.002,
+                 timing: bool = False):
+        self.device = torch.device(device)
+        self.opts = opts or ScfOptions()
+        self.tables = SpeciesTables(elements)
+        self.dtables = DeviceTables(self.tables, self.device)
+        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
+        self.free_streams = deque(self.streams)
+        self.edges = tuple(edges)
+        self.max_batch = tuple(max_batch)
+        self.queues = [deque() for _ in self.edges]
+        self.inflight: list[_Flight] = []
+        self.budget = int(hbm_budget_gb * 2**30)
+        self.forces = forces
+        self.max_wait = max_wait_s
+        self.stats = ServerStats()
+        self.results: dict[int, Request] = {}
+        self.timing = timing
+        self.kernel_log: list[tuple[float, float]] = []  # (ms, algorithmic flops) per SCF launch
+
+    # ------------------------------------------------------------------ intake
+    def submit(self, req: Request) -> None:
+        if req.nao == 0:
+            req.nao = orbital_count(req.numbers)
+        k = int(np.searchsorted(self.edges, req.nao))
+        if k >= len(self.edges):
+            raise ValueError(f"molecule with {req.nao} orbitals exceeds the largest size class")
+        self.queues[k].append(req)
+
+    def pending(self) -> int:
+        return sum(len(q) for q in self.queues)
+
+    def _inflight_bytes(self) -> int:
+        return sum(f.bytes for f in self.inflight)
+
+    # --------------------------------------------------------------- admission
+    def _pick_class(self, now: float) -> int | None:
+        best, best_t = None, None
+        for k, q in enumerate(self.queues):
+            if not q:
+                continue
+            full = len(q) >= self.max_batch[k]
+            waited = now - q[0].arrival >= self.max_wait
+            if not (full or waited or not self.inflight):
+                continue
+            if best is None or q[0].arrival < best_t:
+                best, best_t = k, q[0].arrival
+        return best
+
+    def _admit(self, now: float) -> bool:
+        if not self.free_streams:
+            return False
+        k = self._pick_class(now)
+        if k is None:
+            return False
+        q = self.queues[k]
+        take = [q.popleft() for _ in range(min(self.max_batch[k], len(q)))]
+        lay = BatchLayout([r.numbers for r in take], self.tables, padded=False,
+                          chrg=np.zeros(len(take)))
+        nbytes = 8 * lay.work_doubles()
+        if self.inflight and self._inflight_bytes() + nbytes > self.budget:
+            for r in reversed(take):
+                q.appendleft(r)
+            self.stats.admission_stalls += 1
+            return False
+        stream = self.free_streams.popleft()
+        with torch.cuda.stream(stream):
+            pipe = Pipeline(lay, self.dtables, self.device)
+            pos = torch.from_numpy(np.concatenate([r.positions for r in take]).astype(np.float64))
+            pos = pos.pin_memory().to(self.device, non_blocking=True)
+            pipe.run(pos, self.opts, forces=self.forces, stream=stream, timing=self.timing)
+            ev = torch.cuda.Event()
+            ev.record(stream)
+        self.inflight.append(_Flight(pipe, take, stream, ev, nbytes, now))
+        self.stats.launches += 1
+        self.stats.peak_bytes = max(self.stats.peak_bytes, self._inflight_bytes())
+        return True
+
+    # -------------------------------------------------------------- retirement
+    def _retire(self, f: _Flight) -> list[Request]:
+        f.event.synchronize()
+        pipe = f.pipe
+        if self.timing:
+            ev0, ev1 = pipe.scf_events
+            self.kernel_log.append((ev0.elapsed_time(ev1), pipe.scf_flops()))
+        rec = pipe.records().cpu().numpy()
+        e = pipe.energies().cpu().numpy()
+        g = pipe.gradient().cpu().numpy() if self.forces else None
+        now = time.perf_counter()
+        for b, r in enumerate(f.reqs):
+            r.energy = float(e[b])
+            r.status = int(rec[b, 6])
+            r.iterations = int(rec[b, 5])
+            if g is not None:
+                a0 = int(pipe.L.atom0[b])
+                r.forces = -g[a0:a0 + len(r.numbers)].copy()
+            r.done = now
+            self.results[r.rid] = r
+            self.stats.latencies.append(now - r.arrival)
+        self.stats.completed += len(f.reqs)
+        self.free_streams.append(f.stream)
+        return f.reqs
+
+    def poll(self, block: bool = False) -> list[Request]:
+        out = []
+        keep = []
+        for f in self.inflight:
+            if f.event.query():
+                out += self._retire(f)
+            else:
+                keep.append(f)
+        self.inflight = keep
+        if block and not out and self.inflight:
+            oldest = min(self.inflight, key=lambda f: f.t_launch)
+            self.inflight.remove(oldest)
+            out += self._retire(oldest)
+        return out
+
+    def step(self, now: float | None = None) -> list[Request]:
+        now = time.perf_counter() if now is None else now
+        done = self.poll()
+        while self._admit(now):
+            pass
+        return done
+
+    def drain(self) -> None:
+        while self.pending() or self.inflight:
+            self.step()
+            if self.inflight and not self.free_streams:
+                self.poll(block=True)
+
+    # ------------------------------------------------------------------ driver
+    def run_stream(self, reqs: list[Request], realtime: bool = True) -> dict:
+        """Serve `reqs` (arrival offsets in seconds).  realtime=False submits
+        everything at t0 (saturation throughput)."""
+        t0 = time.perf_counter()
+        pend = deque(sorted(reqs, key=lambda r: r.arrival))
+        for r in pend:
+            r.arrival = t0 + (r.arrival if realtime else 0.0)
+        n = len(pend)
+        while pend or self.pending() or self.inflight:
+            now = time.perf_counter()
+            while pend and pend[0].arrival <= now:
+                self.submit(pend.popleft())
+            self.step(now)
+            if not pend and not self.free_streams:
+                self.poll(block=True)
+            elif pend and not self.pending() and not self.inflight:
+                time.sleep(max(0.0, min(pend[0].arrival - time.perf_counter(), 0.001)))
+        t1 = time.perf_counter()
+        lat = np.array([self.results[r.rid].done - self.results[r.rid].arrival for r in reqs])
+        return {
+            "molecules": n, "seconds": t1 - t0, "molecules_per_s": n / (t1 - t0),
+            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "launches": self.stats.launches, "admission_stalls": self.stats.admission_stalls,
+            "peak_gb": self.stats.peak_bytes / 2**30,
+        }
+
+
+def padded_dropin(reqs: list[Request], batch: int, device="cuda:0", opts: ScfOptions | None = None,
+                  forces: bool = True) -> dict:
+    """Plain padded-batch drop-in: requests in arrival order, `batch` at a
+    time, each batch padded to its largest molecule and run to completion
+    before the next (upstream's batch_mode=1 semantics)."""
+    from . import workloads
+
+    dev = torch.device(device)
+    opts = opts or ScfOptions()
+    tables = SpeciesTables((1, 6, 7, 8))
+    dt = DeviceTables(tables, dev)
+    t0 = time.perf_counter()
+    lat = []
+    for s in range(0, len(reqs), batch):
+        chunk = reqs[s:s + batch]
+        nums, pos = workloads.pad([(r.numbers, r.positions) for r in chunk])
+        mask = nums > 0
+        lay = BatchLayout([n[n > 0] for n in nums], tables, padded=True, chrg=np.zeros(len(chunk)))
+        pipe = Pipeline(lay, dt, dev)
+        p = torch.as_tensor(pos[mask], dtype=torch.float64, device=dev)
+        pipe.run(p, opts, forces=forces)
+        e = pipe.energies().cpu().numpy()
+        if forces:
+            pipe.gradient().cpu()
+        now = time.perf_counter()
+        for r, ei in zip(chunk, e):
+            r.energy = float(ei)
+            lat.append(now - t0)
+    t1 = time.perf_counter()
+    return {"molecules": len(reqs), "seconds": t1 - t0, "molecules_per_s": len(reqs) / (t1 - t0)}

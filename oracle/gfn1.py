@@ -225,16 +225,13 @@ def shell_pair_overlap(li, lj, ai, ci, aj, cj, rvec, grad=False):
     return (blk, dblk) if grad else blk
 
 
-def _overlap_impl(bas: Basis, positions: torch.Tensor, grad: bool):
+def _overlap_impl_loop(bas: Basis, positions: torch.Tensor, grad: bool):
+    """Shell-pair loop version (kept as a cross-check of the batched one)."""
     pos = positions.to(DTYPE)
     n = bas.nao
     S = torch.zeros(n, n, dtype=DTYPE)
     dS = torch.zeros(n, n, 3, dtype=DTYPE)
-    off = []
-    o = 0
-    for l in bas.shell_l:
-        off.append(o)
-        o += 2 * l + 1
+    off = _shell_offsets(bas)
     for i in range(bas.nsh):
         for j in range(bas.nsh):
             A, B = bas.shell_atom[i], bas.shell_atom[j]
@@ -252,6 +249,93 @@ def _overlap_impl(bas: Basis, positions: torch.Tensor, grad: bool):
             else:
                 blk = r
             S[off[i]:off[i] + 2 * li + 1, off[j]:off[j] + 2 * lj + 1] = blk
+    S.fill_diagonal_(1.0)  # overlap.py:241-243
+    return S, dS
+
+
+def _shell_offsets(bas: Basis):
+    off, o = [], 0
+    for l in bas.shell_l:
+        off.append(o)
+        o += 2 * l + 1
+    return off
+
+
+def _overlap_impl(bas: Basis, positions: torch.Tensor, grad: bool):
+    """Batched over all shell pairs of one (l_i, l_j) type at a time, as
+    upstream batches md_explicit over the instances of a pair type
+    (impls/overlap.py:203-233)."""
+    pos = positions.to(DTYPE)
+    n = bas.nao
+    S = torch.zeros(n, n, dtype=DTYPE)
+    dS = torch.zeros(n, n, 3, dtype=DTYPE)
+    nsh = bas.nsh
+    off = torch.tensor(_shell_offsets(bas))
+    sl = torch.tensor(bas.shell_l)
+    sa = torch.tensor(bas.shell_atom)
+    npmax = max(len(a) for a in bas.shell_alpha)
+    al = torch.ones(nsh, npmax, dtype=DTYPE)
+    co = torch.zeros(nsh, npmax, dtype=DTYPE)
+    for k, (a, c) in enumerate(zip(bas.shell_alpha, bas.shell_coeff)):
+        al[k, : len(a)] = a
+        co[k, : len(c)] = c
+    ii, jj = torch.meshgrid(torch.arange(nsh), torch.arange(nsh), indexing="ij")
+    ii, jj = ii.reshape(-1), jj.reshape(-1)
+    rv = pos[sa[ii]] - pos[sa[jj]]
+    dist = torch.linalg.norm(rv, dim=-1)
+    keep = (dist > PAD_DIST) & (dist < INTCUTOFF)
+    ii, jj, rv = ii[keep], jj[keep], rv[keep]
+    for li in (0, 1):
+        for lj in (0, 1):
+            sel = (sl[ii] == li) & (sl[jj] == lj)
+            if not bool(sel.any()):
+                continue
+            i, j, R = ii[sel], jj[sel], rv[sel]
+            a = al[i][:, :, None]
+            b = al[j][:, None, :]
+            p = a + b
+            mu = a * b / p
+            r2 = (R * R).sum(-1)[:, None, None]
+            s00 = (math.pi / p) ** 1.5 * torch.exp(-mu * r2) * co[i][:, :, None] * co[j][:, None, :]
+            pa, pb = -(b / p), a / p
+            ni, nj = 2 * li + 1, 2 * lj + 1
+            blk = torch.zeros(len(i), ni, nj, dtype=DTYPE)
+            dblk = torch.zeros(len(i), 3, ni, nj, dtype=DTYPE)
+            for mi in range(ni):
+                for mj in range(nj):
+                    x = _PCOMP[mi] if li else None
+                    y = _PCOMP[mj] if lj else None
+                    Rx = R[:, x, None, None] if li else None
+                    Ry = R[:, y, None, None] if lj else None
+                    if li == 0 and lj == 0:
+                        f = torch.ones_like(p)
+                    elif lj == 0:
+                        f = pa * Rx
+                    elif li == 0:
+                        f = pb * Ry
+                    else:
+                        f = pa * pb * Rx * Ry + (0.5 / p if x == y else 0.0)
+                    blk[:, mi, mj] = (f * s00).sum((-2, -1))
+                    if grad:
+                        for k in range(3):
+                            df = torch.zeros_like(p)
+                            if li and not lj and k == x:
+                                df = pa.expand_as(p)
+                            elif lj and not li and k == y:
+                                df = pb.expand_as(p)
+                            elif li and lj:
+                                if k == x:
+                                    df = df + pa * pb * Ry
+                                if k == y:
+                                    df = df + pa * pb * Rx
+                            Rk = R[:, k, None, None]
+                            dblk[:, k, mi, mj] = ((df - 2.0 * mu * Rk * f) * s00).sum((-2, -1))
+            rows = off[i][:, None, None] + torch.arange(ni)[None, :, None]
+            cols = off[j][:, None, None] + torch.arange(nj)[None, None, :]
+            rows, cols = rows.expand(-1, ni, nj), cols.expand(-1, ni, nj)
+            S[rows, cols] = blk
+            if grad:
+                dS[rows, cols] = dblk.permute(0, 2, 3, 1)
     S.fill_diagonal_(1.0)  # overlap.py:241-243
     return S, dS
 
@@ -308,33 +392,37 @@ def coordination_number_grad(numbers, positions) -> torch.Tensor:
 # Core Hamiltonian (a9): _src/xtb/base.py:251-360, _src/xtb/gfn1.py:66-165
 # ---------------------------------------------------------------------------
 def _shell_pair_k(bas: Basis, par: dict) -> torch.Tensor:
-    """K_ab: hscale x kpair x X(EN) for valence pairs, hscale else."""
+    """K_ab: hscale x kpair x X(EN) for valence pairs, hscale else
+    (xtb/gfn1.py:66-165, xtb/base.py:316-330).  Evaluated per pair of
+    (element, shell) kinds, then spread to the molecule's shells."""
     h = par["hamiltonian"]
     lab = "spdf"
     kpol = h["kpol"]
     shell = h["shell"]
     sym_of = {e["z"]: s for s, e in par["elements"].items()}
     en_of = {e["z"]: e["en"] for e in par["elements"].values()}
-    nsh = bas.nsh
-    K = torch.zeros(nsh, nsh, dtype=DTYPE)
-    for i in range(nsh):
-        for j in range(nsh):
-            li, lj = bas.shell_l[i], bas.shell_l[j]
-            vi, vj = bas.shell_valence[i], bas.shell_valence[j]
+    kinds = {}
+    kind_of_shell = []
+    for s in range(bas.nsh):
+        z = bas.numbers[bas.shell_atom[s]]
+        key = (z, bas.shell_l[s], bas.shell_valence[s])
+        kind_of_shell.append(kinds.setdefault(key, len(kinds)))
+    kl = list(kinds)
+    Kk = torch.zeros(len(kl), len(kl), dtype=DTYPE)
+    for a, (za, li, vi) in enumerate(kl):
+        for b, (zb, lj, vj) in enumerate(kl):
             kii = shell.get(lab[li] * 2, 1.0) if vi else kpol
             kjj = shell.get(lab[lj] * 2, 1.0) if vj else kpol
             if vi and vj:
-                key1, key2 = lab[li] + lab[lj], lab[lj] + lab[li]
-                hs = shell.get(key1, shell.get(key2, 0.5 * (kii + kjj)))
-                za = bas.numbers[bas.shell_atom[i]]
-                zb = bas.numbers[bas.shell_atom[j]]
+                hs = shell.get(lab[li] + lab[lj], shell.get(lab[lj] + lab[li], 0.5 * (kii + kjj)))
                 kp = h["kpair"].get(f"{sym_of[za]}-{sym_of[zb]}",
                                     h["kpair"].get(f"{sym_of[zb]}-{sym_of[za]}", 1.0))
                 x = 1.0 + h["enscale"] * (en_of[za] - en_of[zb]) ** 2
-                K[i, j] = hs * kp * x
+                Kk[a, b] = hs * kp * x
             else:
-                K[i, j] = 0.5 * (kii + kjj)
-    return K
+                Kk[a, b] = 0.5 * (kii + kjj)
+    ks = torch.tensor(kind_of_shell)
+    return Kk[ks][:, ks]
 
 
 def _hamiltonian_pieces(bas: Basis, positions, cn, par):

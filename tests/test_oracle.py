@@ -138,3 +138,13 @@ def test_fermi_closed_form():
     assert float(f[0]) == pytest.approx(1.0, abs=1e-15)
     ef = -0.1 + 0.5e-4  # symmetric pair around the midpoint
     assert float(f[1]) == pytest.approx(1.0 / (1.0 + math.exp((-0.1 - ef) / kt)), rel=1e-6)
+
+
+def test_batched_overlap_equals_pair_loop(fx):
+    z, p = mol(fx, "lys_xao")
+    bas = og.build_basis(z)
+    pos = torch.tensor(p)
+    S1, d1 = og._overlap_impl(bas, pos, True)
+    S2, d2 = og._overlap_impl_loop(bas, pos, True)
+    assert (S1 - S2).abs().max() < 1e-15
+    assert (d1 - d2).abs().max() < 1e-15
