@@ -23,7 +23,7 @@ MOL_FERMI = 2
 MOL_NONFINITE = 4
 MOL_JACOBI = 8
 
-OUT_STRIDE = 16
+OUT_STRIDE = 32
 OUT_ESCF, OUT_EH0, OUT_EES2, OUT_EES3, OUT_FENERGY = 0, 1, 2, 3, 4
 OUT_ITER, OUT_STATUS, OUT_EF_A, OUT_EF_B, OUT_EREP, OUT_SWEEPS = 5, 6, 7, 8, 9, 10
 

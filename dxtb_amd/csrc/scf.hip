@@ -67,19 +67,24 @@ struct Shared {
     double *red, *cs, *skey, *ev, *ftot, *qsh, *vsh, *gq, *qat, *big;
     int* sidx;
     int npow, big_cap;
+    double* tacc;  // per-thread stage timers (meaningful on thread 0)
 };
+
+// device wall clock (constant 100 MHz) in microseconds
+__device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 
 // ---------------------------------------------------------------------------
 // eigen step: A (full, in mol.F) -> eigenvalues in sh.ev, rotations applied
 // to mol.Ct.  Returns Jacobi sweeps (-1 on failure).
 // ---------------------------------------------------------------------------
-__device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_sweeps, double rel_tol) {
+__device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_sweeps, double rel_tol,
+                                     double loose) {
     const int n = M.n;
     // scale for the absolute tolerance
     double dmax = 0.0;
     for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, fabs(M.F[(size_t)i * n + i]));
     dmax = wg_max(dmax, sh.red);
-    const double tol = rel_tol * fmax(dmax, 1.0);
+    const double tol = fmax(rel_tol * fmax(dmax, 1.0), loose);
     const int npk = n * (n + 1) / 2;
     int sweeps;
     if (npk <= sh.big_cap) {
@@ -100,6 +105,11 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
         if (cl) {
             for (int idx = threadIdx.x; idx < n * n; idx += NT) M.Ct[idx] = cbuf[idx];
         }
+    } else if (sh.big_cap >= NBLOCK_LDS) {
+        // block Jacobi; T is free scratch here and holds the subproblem rotations
+        sweeps = wg_block_jacobi(M.F, n, RowsGlobal{M.Ct, n}, M.T, max_sweeps, tol, sh.big, sh.red,
+                                 sh.tacc);
+        for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = M.F[(size_t)i * n + i];
     } else {
         SymFull A{M.F, n};
         sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red);
@@ -109,33 +119,69 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
     return sweeps;
 }
 
-// Loewdin orthogonaliser: Ct <- X = S^-1/2 (symmetric, so rows == columns).
+// Loewdin orthogonaliser Ct <- X = S^-1/2 (symmetric, so rows == columns) by
+// the coupled Newton-Schulz iteration on matrix cores:
+//   Y0 = S/c, Z0 = I;  T = (3I - Z Y)/2;  Y <- Y T;  Z <- T Z
+// Y -> (S/c)^1/2 and Z -> (S/c)^-1/2 quadratically once ||I - S/c|| < 1; c is a
+// Gershgorin bound on the largest eigenvalue of S.  X = Z / sqrt(c).
 __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* status) {
     const int n = M.n;
     const size_t nn = (size_t)n * n;
+    double rs = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        double a = 0.0;
+        for (int j = 0; j < n; ++j) a += fabs(M.S[(size_t)i * n + j]);
+        rs = fmax(rs, a);
+    }
+    const double c = wg_max(rs, sh.red) * 1.0001;
+    double* Y = M.F;
+    double* Z = M.Ct;
+    double* T = M.T;
+    double* W = M.P;
     for (size_t e = threadIdx.x; e < nn; e += NT) {
-        M.F[e] = M.S[e];
-        const int r = (int)(e / n), c = (int)(e % n);
-        M.Ct[e] = (r == c) ? 1.0 : 0.0;
+        const int r = (int)(e / n), q = (int)(e % n);
+        Y[e] = M.S[e] / c;
+        Z[e] = (r == q) ? 1.0 : 0.0;
     }
     __syncthreads();
-    int sw = eig_step(M, sh, 60, 1e-15);
-    if (sw < 0) *status |= XTB_MOL_JACOBI;
-    // T[k][mu] = s_k^-1/2 U[mu][k]
-    double bad = 0.0;
-    for (size_t e = threadIdx.x; e < nn; e += NT) {
-        const int k = (int)(e / n);
-        const double s = sh.ev[k];
-        if (!(s > 0.0)) bad = 1.0;
-        M.T[e] = M.Ct[e] / sqrt(s);
+    int it = 0;
+    int extra = -1;  // iterations left once in the quadratic regime
+    for (; it < 60; ++it) {
+        gemm<false, false>(n, n, n, -0.5, Z, n, Y, n, 0.0, T, n, sh.big);  // -(Z Y)/2
+        double err = 0.0;
+        for (size_t e = threadIdx.x; e < nn; e += NT) {
+            const int r = (int)(e / n), q = (int)(e % n);
+            const double v = T[e] + ((r == q) ? 1.5 : 0.0);
+            err = fmax(err, fabs(v - ((r == q) ? 1.0 : 0.0)));
+            T[e] = v;
+        }
+        err = wg_max(err, sh.red);
+        if (extra == 0 || err < 1e-15) break;
+        if (extra > 0) --extra;
+        else if (err < 1e-9) extra = 1;  // quadratic: one more step reaches rounding level
+        gemm<false, false>(n, n, n, 1.0, Y, n, T, n, 0.0, W, n, sh.big);  // Y T
+        gemm<false, false>(n, n, n, 1.0, T, n, Z, n, 0.0, Y, n, sh.big);  // T Z (into Y's buffer)
+        // rotate buffers: Y <- W, Z <- (T Z) held in Y's buffer
+        double* oldZ = Z;
+        Z = Y;
+        Y = W;
+        W = oldZ;
     }
-    bad = wg_max(bad, sh.red);
-    if (bad > 0.0) *status |= XTB_MOL_NONFINITE;
-    // X = U s^-1/2 U^T  ->  F
-    wg_gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
-    for (size_t e = threadIdx.x; e < nn; e += NT) M.Ct[e] = M.F[e];
+    if (it >= 60) *status |= XTB_MOL_JACOBI;
+    const double sc = 1.0 / sqrt(c);
+    for (size_t e = threadIdx.x; e < nn; e += NT) M.Ct[e] = Z[e] * sc;  // Z may alias Ct
     __syncthreads();
-    return sw < 0 ? 0 : sw;
+    // symmetrise (rounding)
+    for (size_t e = threadIdx.x; e < nn; e += NT) {
+        const int r = (int)(e / n), q = (int)(e % n);
+        if (q > r) {
+            const double a = 0.5 * (M.Ct[e] + M.Ct[(size_t)q * n + r]);
+            M.Ct[e] = a;
+            M.Ct[(size_t)q * n + r] = a;
+        }
+    }
+    __syncthreads();
+    return 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -201,7 +247,7 @@ __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, doubl
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
                         const xtb_params& par, const xtb_scf_opts& o, const double* vin,
-                        double* vout, int* status, double* ef) {
+                        double* vout, int* status, double* ef, double loose) {
     const int n = M.n;
     const size_t nn = (size_t)n * n;
     // Fock build (scf/base.py:817-859)
@@ -211,9 +257,14 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     }
     __syncthreads();
     // T = F C ; A = C^T T  -> F
-    wg_gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
-    wg_gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
-    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol);
+    double t0 = now_us();
+    gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
+    gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
+    double t1 = now_us();
+    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose);
+    double t2 = now_us();
+    sh.tacc[1] += t1 - t0;
+    sh.tacc[2] += t2 - t1;
     if (sw < 0) *status |= XTB_MOL_JACOBI;
     // sort eigenvalues
     for (int j = threadIdx.x; j < sh.npow; j += NT) {
@@ -232,7 +283,9 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     // P = C diag(f) C^T
     for (size_t e = threadIdx.x; e < nn; e += NT) M.T[e] = sh.ftot[e / n] * M.Ct[e];
     __syncthreads();
-    wg_gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.P, n, sh.big);
+    t0 = now_us();
+    gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.P, n, sh.big);
+    sh.tacc[1] += now_us() - t0;
     // Mulliken charges and orbital energies (scf/base.py:764-815), one wave per row
     {
         const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
@@ -308,7 +361,7 @@ struct Mixer {
 
 __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh, const xtb_scf_opts& o,
                          double* hx, double* hf, const double* xnew, const double* xold,
-                         double* xout) {
+                         double* xout, double* resid_inf) {
     const int n = M.n, G = mx.G;
     auto X = [&](int h) { return hx + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
     auto Fh = [&](int h) { return hf + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
@@ -331,6 +384,7 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
     l2 = wg_sum(l2, sh.red);
     linf = wg_max(linf, sh.red);
     const bool conv = (sqrt(l2) < o.x_atol) && (linf < o.x_atol_max);
+    *resid_inf = linf;
     double theta[5] = {0, 0, 0, 0, 0};
     int nn = 0;
     if (mx.step > gen) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
@@ -473,7 +527,11 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         return;
     }
     int status = 0;
+    double tacc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    sh.tacc = tacc;
+    const double tstart = now_us();
     int sweeps = ortho_setup(M, sh, &status);
+    tacc[0] = now_us() - tstart;
     double ef[2] = {0.0, 0.0};
 
     // guess: zero charges -> zero potential (scf/base.py:363-420)
@@ -482,26 +540,30 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     Mixer mx{0, 0, o.generations + 1};
     int iters = 0;
     // unrolling/default.py:80-100: evaluate the guess, optionally mix it
-    sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef);
+    // Inexact SCF: each step's eigensolve is converged only as far as the
+    // current SCF residual warrants (1e-2 of the last |dv|_inf); the last
+    // step, from which every reported quantity comes, to full tolerance.
+    double resid = 1e-2;
+    sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, 1e-4);
     ++iters;
     if (o.mix_guess) {
-        anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot);
+        anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot, &resid);
     } else {
         for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = M.vnew[i];
         __syncthreads();
     }
     bool converged = false;
     for (int it = 0; it < o.maxiter; ++it) {
-        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef);
+        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, 1e-2 * resid);
         ++iters;
-        const bool c = anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot);
+        const bool c = anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot, &resid);
         if (c) { converged = true; break; }
     }
     if (!converged && o.maxiter > 0) status |= XTB_MOL_MAXITER;
     if (o.maxiter == 0) converged = true;
     // converged_to_charges (scf/base.py:472-513): density of the last SCF
     // output potential; the potential of its charges goes to st.pot
-    sweeps += scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef);
+    sweeps += scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0);
 
     // energies (scf/base.py:514-623)
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -533,7 +595,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
             M.F[e] = sh.ftot[k] * sh.ev[k] * M.Ct[e];
         }
         __syncthreads();
-        wg_gemm<true, false>(M.n, M.n, M.n, 1.0, M.Ct, M.n, M.F, M.n, 0.0, M.T, M.n, sh.big);
+        gemm<true, false>(M.n, M.n, M.n, 1.0, M.Ct, M.n, M.F, M.n, 0.0, M.T, M.n, sh.big);
     }
     if (threadIdx.x == 0) {
         const double escf = acc[0] + acc[1] + acc[2];
@@ -548,6 +610,13 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_EFERMI_A] = ef[0];
         out[XTB_OUT_EFERMI_B] = ef[1];
         out[XTB_OUT_SWEEPS] = (double)sweeps;
+        out[XTB_OUT_T_ORTHO] = tacc[0];
+        out[XTB_OUT_T_GEMM] = tacc[1];
+        out[XTB_OUT_T_EIGEN] = tacc[2];
+        out[XTB_OUT_T_TOTAL] = now_us() - tstart;
+        out[XTB_OUT_T_BSUB] = tacc[4];
+        out[XTB_OUT_T_BROW] = tacc[5];
+        out[XTB_OUT_T_BCOL] = tacc[6];
     }
 }
 
@@ -563,9 +632,10 @@ extern "C" int32_t xtb_scf_lds_bytes(const xtb_batch* batch, int32_t* bytes, int
     const int want_c = n * (n + 1) / 2 + n * n;
     const int want_a = n * (n + 1) / 2;
     const int avail = budget - L.total_fixed;
-    int cap = NGEMM_LDS;
+    int cap = NGEMM_MFMA_LDS;
     if (want_c <= avail) cap = want_c > cap ? want_c : cap;
     else if (want_a <= avail) cap = want_a > cap ? want_a : cap;
+    else cap = NBLOCK_LDS > cap ? NBLOCK_LDS : cap;  // block Jacobi
     if (L.total_fixed + cap > budget) return XTB_ETOOBIG;
     *big_cap = cap;
     *bytes = (L.total_fixed + cap) * 8;

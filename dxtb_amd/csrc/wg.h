@@ -170,6 +170,111 @@ __device__ __noinline__ void wg_gemm(int M, int N, int K, double alpha, const do
 }
 
 // ---------------------------------------------------------------------------
+// FP64 matrix cores.  v_mfma_f64_16x16x4_f64: D(16x16) += A(16x4) B(4x16);
+// lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
+// D[l/16 + 4r][l%16] in element r of its 4-vector (checked on gfx950).  So
+// element r of an output fragment is exactly the B operand of k-step r of a
+// following product: fragments chain without a trip through LDS.
+// ---------------------------------------------------------------------------
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ inline v4d mfma16(double a, double b, v4d c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ inline v4d v4zero() {
+    v4d z = {0.0, 0.0, 0.0, 0.0};
+    return z;
+}
+
+// C = alpha op(A) op(B) + beta C on matrix cores.  Workgroup tile TILE x TILE
+// (TILE = 64 or 128), 8 waves as 2 (rows) x 4 (cols), k panels of 16 staged in
+// LDS as As[k][m], Bs[k][n].  `lds` holds >= 2*16*(TILE+4) doubles.
+template <bool TA, bool TB, int TILE>
+__device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const double* __restrict__ A,
+                                          int lda, const double* __restrict__ B, int ldb, double beta,
+                                          double* __restrict__ C, int ldc, double* lds) {
+    static_assert(NT == 512, "wave layout assumes 8 waves");
+    constexpr int LDT = TILE + 4;
+    constexpr int E = TILE * 16 / NT;  // staged elements per thread and operand
+    constexpr int FM = TILE / 32, FN = TILE / 64;
+    double* As = lds;
+    double* Bs = lds + 16 * LDT;
+    const int t = threadIdx.x;
+    const int w = t / WAVE, lane = t % WAVE, li = lane % 16, lg = lane / 16;
+    const int wm = w / 4, wn = w % 4;
+    const int tm = (M + TILE - 1) / TILE, tn = (N + TILE - 1) / TILE;
+    for (int tile = 0; tile < tm * tn; ++tile) {
+        const int m0 = (tile / tn) * TILE, n0 = (tile % tn) * TILE;
+        v4d acc[FM][FN];
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) acc[i][j] = v4zero();
+        for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int m, k;
+                if (TA) { k = t / (TILE / E); m = (t % (TILE / E)) * E + e; }
+                else { m = t / (16 / E); k = (t % (16 / E)) * E + e; }
+                const int gm = m0 + m, gk = k0 + k;
+                double v = 0.0;
+                if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
+                As[k * LDT + m] = v;
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int n, k;
+                if (TB) { n = t / (16 / E); k = (t % (16 / E)) * E + e; }
+                else { k = t / (TILE / E); n = (t % (TILE / E)) * E + e; }
+                const int gn = n0 + n, gk = k0 + k;
+                double v = 0.0;
+                if (gn < N && gk < K) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
+                Bs[k * LDT + n] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < 4; ++kk) {
+                double a[FM], b[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDT + wm * (TILE / 2) + i * 16 + li];
+#pragma unroll
+                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDT + wn * (TILE / 4) + j * 16 + li];
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int j = 0; j < FN; ++j)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int gm = m0 + wm * (TILE / 2) + i * 16 + lg + 4 * r;
+                    const int gn = n0 + wn * (TILE / 4) + j * 16 + li;
+                    if (gm < M && gn < N) {
+                        double* c = C + (size_t)gm * ldc + gn;
+                        *c = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * *c);
+                    }
+                }
+    }
+    __syncthreads();
+}
+
+constexpr int NGEMM_MFMA_LDS = 2 * 16 * (128 + 4);
+
+// dispatch: matrix-core GEMM with a tile matched to the size
+template <bool TA, bool TB>
+__device__ inline void gemm(int M, int N, int K, double alpha, const double* A, int lda,
+                            const double* B, int ldb, double beta, double* C, int ldc, double* lds) {
+    if (M <= 64 && N <= 64) wg_gemm_mfma<TA, TB, 64>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
+    else wg_gemm_mfma<TA, TB, 128>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
+}
+
+// ---------------------------------------------------------------------------
 // Symmetric matrix storage used by the Jacobi eigensolver.  Only the upper
 // triangle (i <= j) is referenced.  Packed form lives in LDS, full form in
 // global memory.
@@ -295,6 +400,345 @@ __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, doub
                 vq = s * a + c * b;
             }
             __syncthreads();
+        }
+        ++sweep;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Block Jacobi for matrices that do not fit in LDS.
+//
+// Indices are grouped in blocks of JB = 16; a round of the round-robin
+// ordering pairs the blocks, and every pair (I, J) defines a subproblem on
+// the <= 32 indices of I u J.  One wavefront diagonalises its subproblem in
+// LDS with scalar cyclic Jacobi (no workgroup barrier inside), producing an
+// orthogonal Q_k.  The workgroup then applies all Q_k of the round to A
+// (rows, then columns) and to the vectors Ct, streaming each element once
+// per phase and reusing every loaded value 32 times.  Per sweep this moves
+// ~3 n^3 bytes instead of the ~32 n^3 of the scalar method.
+// ---------------------------------------------------------------------------
+constexpr int JB = 16;
+constexpr int JS = 2 * JB;                  // subproblem dimension
+constexpr int JPK = JS * (JS + 1) / 2;      // packed subproblem
+constexpr int JWAVE_LDS = JPK + JS * JS + JS;  // M (packed), Q, c/s
+constexpr int NBLOCK_LDS = NWAVE * JWAVE_LDS + 64;  // + identity flags
+
+__device__ inline void wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ inline double& pk(double* M, int i, int j) {
+    if (i > j) { int t = i; i = j; j = t; }
+    return M[i * JS - (i * (i - 1)) / 2 + (j - i)];
+}
+
+// Global index of slot i (0..31) of the subproblem of block pair (I, J):
+// slots 0..15 are block I, 16..31 block J; -1 where the index is beyond n
+// (partial last block, or the dummy block of an odd block count).
+__device__ inline int slot_index(int I, int J, int i, int n) {
+    const int g = (i < JB) ? I * JB + i : J * JB + (i - JB);
+    return g < n ? g : -1;
+}
+
+// Round-robin pair of a 32-slot subproblem (constant modulus).
+__device__ inline void rr32(int r, int k, int& p, int& q) {
+    if (k == 0) { p = 31; q = r; }
+    else { p = (r + k) % 31; q = (r - k + 31) % 31; }
+}
+
+// Wave-level cyclic Jacobi on the packed JS x JS matrix M (slots whose
+// rows/columns are zero are never rotated); Q (row-major JS x JS) accumulates
+// the rotations so that M_final = Q^T M_0 Q.  Returns 1 if any rotation was
+// applied, 0 if M was already diagonal to `tol` (Q = I).
+__device__ __noinline__ int wave_jacobi_sub(double* M, double* Q, double* cs, double tol, int max_sweeps) {
+    constexpr int NPAIR = JS / 2;                   // 16 pairs per round
+    constexpr int NBLK = NPAIR * (NPAIR + 1) / 2;   // 136 2x2 blocks with P <= R
+    const int lane = threadIdx.x % WAVE;
+    for (int e = lane; e < JS * JS; e += WAVE) Q[e] = ((e / JS) == (e % JS)) ? 1.0 : 0.0;
+    wave_sync();
+    const double skip = 1e-3 * tol;
+    int rotated = 0;
+    for (int sw = 0; sw < max_sweeps; ++sw) {
+        double off = 0.0;
+        for (int e = lane; e < JS * JS; e += WAVE) {
+            const int i = e / JS, j = e % JS;
+            if (j > i) off = fmax(off, fabs(pk(M, i, j)));
+        }
+        off = wave_max(off);
+        if (off <= tol) break;
+        rotated = 1;
+        for (int r = 0; r < JS - 1; ++r) {
+            if (lane < NPAIR) {
+                int p, q;
+                rr32(r, lane, p, q);
+                double c = 1.0, s = 0.0;
+                const double apq = pk(M, p, q);
+                if (fabs(apq) > skip) {
+                    const double theta = (pk(M, q, q) - pk(M, p, p)) / (2.0 * apq);
+                    double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
+                    if (theta < 0.0) t = -t;
+                    c = 1.0 / sqrt(fma(t, t, 1.0));
+                    s = t * c;
+                }
+                cs[lane] = c;
+                cs[JB + lane] = s;
+            }
+            wave_sync();
+            for (int e = lane; e < NBLK; e += WAVE) {
+                // triangular index e -> (P, R), P <= R
+                int P = 0, rem = e;
+                while (rem >= NPAIR - P) { rem -= NPAIR - P; ++P; }
+                const int R = P + rem;
+                const double sp = cs[JB + P], sr = cs[JB + R];
+                if (sp == 0.0 && sr == 0.0) continue;
+                int p, q, x, y;
+                rr32(r, P, p, q);
+                rr32(r, R, x, y);
+                const double cp = cs[P], cr = cs[R];
+                if (P == R) {
+                    const double apq = pk(M, p, q);
+                    const double t = sp / cp;
+                    pk(M, p, p) -= t * apq;
+                    pk(M, q, q) += t * apq;
+                    pk(M, p, q) = 0.0;
+                } else {
+                    const double m00 = pk(M, p, x), m01 = pk(M, p, y);
+                    const double m10 = pk(M, q, x), m11 = pk(M, q, y);
+                    const double r00 = cp * m00 - sp * m10, r01 = cp * m01 - sp * m11;
+                    const double r10 = sp * m00 + cp * m10, r11 = sp * m01 + cp * m11;
+                    pk(M, p, x) = r00 * cr - r01 * sr;
+                    pk(M, p, y) = r00 * sr + r01 * cr;
+                    pk(M, q, x) = r10 * cr - r11 * sr;
+                    pk(M, q, y) = r10 * sr + r11 * cr;
+                }
+            }
+            for (int e = lane; e < NPAIR * JS; e += WAVE) {
+                const int k = e / JS, row = e % JS;
+                const double s = cs[JB + k];
+                if (s == 0.0) continue;
+                int p, q;
+                rr32(r, k, p, q);
+                const double c = cs[k];
+                const double a = Q[row * JS + p], b = Q[row * JS + q];
+                Q[row * JS + p] = c * a - s * b;
+                Q[row * JS + q] = s * a + c * b;
+            }
+            wave_sync();
+        }
+    }
+    return rotated;
+}
+
+// Two-sided tile update on matrix cores (one wavefront):
+// A[P_k, P_l] <- Q_k^T A[P_k, P_l] Q_l, written to both triangles.
+// A is symmetric before the round, so the tile is read as A[P_l, P_k]
+// (row segments, coalesced).  Y = A_kl Q_l stays in registers and feeds the
+// second product as its B operand (element r = k-step r).
+__device__ __noinline__ void tile_update(double* A, int n, const double* Qk, const double* Ql,
+                                         int Ik, int Jk, int Il, int Jl, int idk, int idl) {
+    const int lane = threadIdx.x % WAVE, li = lane % 16, lg = lane / 16;
+    int gk[2], gl_[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+        gk[b] = slot_index(Ik, Jk, b * 16 + li, n);
+        gl_[b] = slot_index(Il, Jl, b * 16 + li, n);
+    }
+    v4d y[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) y[i][j] = v4zero();
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int kp = kb * 16 + lg + 4 * kk;
+            const int g = slot_index(Il, Jl, kp, n);
+            double a[2], bq[2];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm)
+                a[bm] = (g >= 0 && gk[bm] >= 0) ? A[(size_t)g * n + gk[bm]] : 0.0;
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn)
+                bq[bn] = idl ? ((kp == bn * 16 + li) ? 1.0 : 0.0) : Ql[kp * JS + bn * 16 + li];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+                for (int bn = 0; bn < 2; ++bn) y[bm][bn] = mfma16(a[bm], bq[bn], y[bm][bn]);
+        }
+    v4d z[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) z[i][j] = v4zero();
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int k = kb * 16 + lg + 4 * r;
+            double a[2];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm)
+                a[bm] = idk ? ((k == bm * 16 + li) ? 1.0 : 0.0) : Qk[k * JS + bm * 16 + li];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+                for (int bn = 0; bn < 2; ++bn) z[bm][bn] = mfma16(a[bm], y[kb][bn][r], z[bm][bn]);
+        }
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int gi = slot_index(Ik, Jk, bm * 16 + lg + 4 * r, n);
+            if (gi < 0) continue;
+#pragma unroll
+            for (int bn = 0; bn < 2; ++bn) {
+                const int gj = gl_[bn];
+                if (gj < 0) continue;
+                A[(size_t)gi * n + gj] = z[bm][bn][r];
+                A[(size_t)gj * n + gi] = z[bm][bn][r];
+            }
+        }
+}
+
+// Vector update on matrix cores (one wavefront): rows P_k of the row-major
+// matrix X (stride ld), columns c0..c0+63:  X[P_k, c] <- Q_k^T X[P_k, c].
+__device__ __noinline__ void rows_update(double* X, int n, size_t ld, int ncol, const double* Qk,
+                                         int Ik, int Jk, int c0) {
+    const int lane = threadIdx.x % WAVE, li = lane % 16, lg = lane / 16;
+    v4d acc[2][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = v4zero();
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb)
+#pragma unroll
+        for (int kk = 0; kk < 4; ++kk) {
+            const int kp = kb * 16 + lg + 4 * kk;
+            const int g = slot_index(Ik, Jk, kp, n);
+            double a[2], b[4];
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm) a[bm] = Qk[kp * JS + bm * 16 + li];
+#pragma unroll
+            for (int bn = 0; bn < 4; ++bn) {
+                const int c = c0 + bn * 16 + li;
+                b[bn] = (g >= 0 && c < ncol) ? X[(size_t)g * ld + c] : 0.0;
+            }
+#pragma unroll
+            for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+                for (int bn = 0; bn < 4; ++bn) acc[bm][bn] = mfma16(a[bm], b[bn], acc[bm][bn]);
+        }
+#pragma unroll
+    for (int bm = 0; bm < 2; ++bm)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int g = slot_index(Ik, Jk, bm * 16 + lg + 4 * r, n);
+            if (g < 0) continue;
+#pragma unroll
+            for (int bn = 0; bn < 4; ++bn) {
+                const int c = c0 + bn * 16 + li;
+                if (c < ncol) X[(size_t)g * ld + c] = acc[bm][bn][r];
+            }
+        }
+}
+
+// A: full symmetric n x n (row-major, global); Ct: vectors (rows, global);
+// Qbuf: global scratch >= (nblocks/2 + 1) * JS * JS doubles.  Returns sweeps
+// or -1 when max_sweeps were spent.
+template <class CAcc>
+__device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Qbuf, int max_sweeps, double tol,
+                                            double* lds, double* red, double* tacc) {
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    double* M = lds + w * JWAVE_LDS;
+    double* Q = M + JPK;
+    double* cs = Q + JS * JS;
+    const int nb = (n + JB - 1) / JB;
+    const int nb2 = nb + (nb & 1);
+    const int npair = nb2 / 2;
+    const int ntile = npair * (npair - 1) / 2;
+    const int nch = (n + WAVE - 1) / WAVE;
+    int* ident = reinterpret_cast<int*>(lds + NWAVE * JWAVE_LDS);  // per pair: Q == I
+    // symmetrise from the upper triangle
+    for (int e = threadIdx.x; e < n * n; e += NT) {
+        const int i = e / n, j = e % n;
+        if (j < i) A[(size_t)i * n + j] = A[(size_t)j * n + i];
+    }
+    __syncthreads();
+    int sweep = 0;
+    for (;;) {
+        double off = 0.0;
+        for (int e = threadIdx.x; e < n * n; e += NT) {
+            const int i = e / n, j = e % n;
+            if (j > i) off = fmax(off, fabs(A[(size_t)i * n + j]));
+        }
+        off = wg_max(off, red);
+        if (off <= tol) return sweep;
+        if (sweep >= max_sweeps) return -1;
+        for (int r = 0; r < nb2 - 1; ++r) {
+            const double t0 = (double)wall_clock64() * 0.01;
+            // 1. subproblems, one wavefront each (invalid slots stay decoupled:
+            //    zero rows/columns are never rotated)
+            for (int k = w; k < npair; k += NWAVE) {
+                int I, J;
+                rr_pair(r, k, nb2, I, J);
+                for (int e = lane; e < JS * JS; e += WAVE) {
+                    const int i = e / JS, j = e % JS;
+                    if (j >= i) {
+                        const int gi = slot_index(I, J, i, n), gj = slot_index(I, J, j, n);
+                        double v = 0.0;
+                        if (gi >= 0 && gj >= 0)
+                            v = A[(size_t)(gi < gj ? gi : gj) * n + (gi < gj ? gj : gi)];
+                        pk(M, i, j) = v;
+                    }
+                }
+                wave_sync();
+                const int rot = wave_jacobi_sub(M, Q, cs, tol, 2);
+                double* qg = Qbuf + (size_t)k * JS * JS;
+                if (lane == 0) ident[k] = rot ? 0 : 1;
+                if (rot) {
+                    for (int e = lane; e < JS * JS; e += WAVE) {
+                        qg[e] = Q[e];
+                        const int i = e / JS, j = e % JS;
+                        const int gi = slot_index(I, J, i, n), gj = slot_index(I, J, j, n);
+                        if (gi >= 0 && gj >= 0) A[(size_t)gi * n + gj] = pk(M, i, j);
+                    }
+                }
+                wave_sync();
+            }
+            __syncthreads();
+            const double t1 = (double)wall_clock64() * 0.01;
+            // 2. off-diagonal tiles of A and the vectors, matrix cores
+            const int nitem = ntile + npair * nch;
+            for (int it = w; it < nitem; it += NWAVE) {
+                if (it < ntile) {
+                    // tile index -> (k, l), k < l
+                    int k = 0, rem = it;
+                    while (rem >= npair - 1 - k) { rem -= npair - 1 - k; ++k; }
+                    const int l = k + 1 + rem;
+                    if (ident[k] && ident[l]) continue;  // both rotations are I
+                    int Ik, Jk, Il, Jl;
+                    rr_pair(r, k, nb2, Ik, Jk);
+                    rr_pair(r, l, nb2, Il, Jl);
+                    tile_update(A, n, Qbuf + (size_t)k * JS * JS, Qbuf + (size_t)l * JS * JS, Ik, Jk,
+                                Il, Jl, ident[k], ident[l]);
+                } else {
+                    const int v = it - ntile;
+                    const int k = v / nch, ch = v % nch;
+                    if (ident[k]) continue;
+                    int Ik, Jk;
+                    rr_pair(r, k, nb2, Ik, Jk);
+                    rows_update(Ct.a, n, (size_t)Ct.ld, n, Qbuf + (size_t)k * JS * JS, Ik, Jk,
+                                ch * WAVE);
+                }
+            }
+            __syncthreads();
+            const double t2 = (double)wall_clock64() * 0.01;
+            tacc[4] += t1 - t0;
+            tacc[5] += t2 - t1;
         }
         ++sweep;
     }

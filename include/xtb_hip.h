@@ -69,7 +69,7 @@ extern "C" {
 #define XTB_MOL_JACOBI 8    /* eigensolver hit its sweep limit */
 
 /* layout of the per-molecule result record */
-#define XTB_OUT_STRIDE 16
+#define XTB_OUT_STRIDE 32
 #define XTB_OUT_ESCF 0      /* E_h0 + E_es2 + E_es3 */
 #define XTB_OUT_EH0 1
 #define XTB_OUT_EES2 2
@@ -81,6 +81,14 @@ extern "C" {
 #define XTB_OUT_EFERMI_B 8
 #define XTB_OUT_EREP 9      /* written by xtb_repulsion */
 #define XTB_OUT_SWEEPS 10   /* total Jacobi sweeps spent */
+/* stage timers of the SCF kernel, microseconds (device wall clock) */
+#define XTB_OUT_T_ORTHO 11  /* orthogonaliser (eigensolve of S) */
+#define XTB_OUT_T_GEMM 12   /* Fock transform, density and W products */
+#define XTB_OUT_T_EIGEN 13  /* Jacobi sweeps of the SCF steps */
+#define XTB_OUT_T_TOTAL 14  /* whole molecule */
+#define XTB_OUT_T_BSUB 15   /* block Jacobi: subproblem solves */
+#define XTB_OUT_T_BROW 16   /* block Jacobi: row-panel updates (A and vectors) */
+#define XTB_OUT_T_BCOL 17   /* block Jacobi: column-panel updates */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {

@@ -86,3 +86,23 @@ def test_padded_layout_matches_ragged(fx):
     b = run_native(mols, padded=True, forces=True)
     assert np.abs(a.energies().cpu().numpy() - b.energies().cpu().numpy()).max() < 1e-10
     assert np.abs(a.gradient().cpu().numpy() - b.gradient().cpu().numpy()).max() < 1e-10
+
+
+def test_large_molecules_block_jacobi_vs_oracle():
+    """n > LDS capacity: exercises the block-Jacobi eigensolver."""
+    rng = np.random.default_rng(5)
+    mols = [W.chain(rng, 40), W.water_cluster(rng, 50), W.chain(rng, 30)]
+    print("case: block jacobi n=" + ",".join(str(len(z)) for z, _ in mols) + " atoms", flush=True)
+    pipe = run_native(mols, forces=True)
+    rec = pipe.records().cpu().numpy()
+    E = pipe.energies().cpu().numpy()
+    G = pipe.gradient().cpu().numpy()
+    assert int(pipe.L.n.max()) > 200
+    for b, (z, p) in enumerate(mols):
+        r = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
+        assert int(rec[b, 6]) == 0, f"status {rec[b, 6]}"
+        assert abs(E[b] - r["energy"]) < 1e-9
+        q = pipe.orbital_charges(b).cpu().numpy()
+        assert np.abs(q - r["charges_orbital"].numpy()).max() < 1e-8
+        a0 = int(pipe.L.atom0[b])
+        assert np.abs(G[a0:a0 + len(z)] - r["gradient"].numpy()).max() < 1e-8
