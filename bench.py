@@ -76,6 +76,10 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-padded", action="store_true")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="serve one chunk only (for rocprofv3 runs), print nothing else")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {a.gpus} is not supported "
@@ -91,12 +95,25 @@ def main() -> int:
 
     log = (lambda *x: None) if a.quiet else (lambda *x: print(*x, file=sys.stderr, flush=True))
     dev = torch.device("cuda:0")
+    if a.copy_probe:
+        x = torch.ones(a.copy_probe * 2**20 // 8, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_mib": a.copy_probe, "bytes_read": x.numel() * 8,
+                          "bytes_written": y.numel() * 8}))
+        return 0
     opts = ScfOptions(x_atol=1e-10, x_atol_max=1e-10, maxiter=300)
     total = (a.warmup + a.steps) * a.chunk + a.latency_requests
     stream = W.config4(seed=4, count=total)
     log(f"stream: {total} requests generated")
 
     server = RaggedServer(dev, opts=opts)
+    if a.profile_run:
+        r = server.run_stream(_requests(stream, 0, a.chunk), realtime=False)
+        print(json.dumps({"profile_run": r}))
+        return 0
     pos = 0
     for w in range(a.warmup):
         server.run_stream(_requests(stream, pos, a.chunk), realtime=False)

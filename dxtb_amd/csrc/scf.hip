@@ -78,7 +78,7 @@ __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 // to mol.Ct.  Returns Jacobi sweeps (-1 on failure).
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_sweeps, double rel_tol,
-                                     double loose) {
+                                     double loose, int inner) {
     const int n = M.n;
     // scale for the absolute tolerance
     double dmax = 0.0;
@@ -108,7 +108,7 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
     } else if (sh.big_cap >= NBLOCK_LDS) {
         // block Jacobi; T is free scratch here and holds the subproblem rotations
         sweeps = wg_block_jacobi(M.F, n, RowsGlobal{M.Ct, n}, M.T, max_sweeps, tol, sh.big, sh.red,
-                                 sh.tacc);
+                                 sh.tacc, inner);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = M.F[(size_t)i * n + i];
     } else {
         SymFull A{M.F, n};
@@ -261,7 +261,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
     gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
     double t1 = now_us();
-    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose);
+    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner);
     double t2 = now_us();
     sh.tacc[1] += t1 - t0;
     sh.tacc[2] += t2 - t1;
@@ -544,7 +544,9 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     // current SCF residual warrants (1e-2 of the last |dv|_inf); the last
     // step, from which every reported quantity comes, to full tolerance.
     double resid = 1e-2;
-    sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, 1e-4);
+    const int sw_first = scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef,
+                                  o.eig_inexact > 0.0 ? 1e-4 : 0.0);
+    sweeps += sw_first;
     ++iters;
     if (o.mix_guess) {
         anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot, &resid);
@@ -554,7 +556,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     }
     bool converged = false;
     for (int it = 0; it < o.maxiter; ++it) {
-        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, 1e-2 * resid);
+        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, o.eig_inexact * resid);
         ++iters;
         const bool c = anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot, &resid);
         if (c) { converged = true; break; }
@@ -563,7 +565,8 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     if (o.maxiter == 0) converged = true;
     // converged_to_charges (scf/base.py:472-513): density of the last SCF
     // output potential; the potential of its charges goes to st.pot
-    sweeps += scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0);
+    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0);
+    sweeps += sw_last;
 
     // energies (scf/base.py:514-623)
     double acc[4] = {0.0, 0.0, 0.0, 0.0};
@@ -617,6 +620,8 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_T_BSUB] = tacc[4];
         out[XTB_OUT_T_BROW] = tacc[5];
         out[XTB_OUT_T_BCOL] = tacc[6];
+        out[XTB_OUT_SWEEPS_FIRST] = (double)sw_first;
+        out[XTB_OUT_SWEEPS_LAST] = (double)sw_last;
     }
 }
 
@@ -649,6 +654,7 @@ extern "C" int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par, const 
     if (batch->nmol <= 0 || batch->max_n <= 0) return XTB_EINVAL;
     if (batch->max_n > 1024) return XTB_ETOOBIG;
     if (opts->generations < 1 || opts->generations > 5) return XTB_EINVAL;
+    if (opts->jacobi_inner < 1 || opts->jacobi_inner > 20 || opts->eig_inexact < 0.0) return XTB_EINVAL;
     if (!state->F || !state->C || !state->T || !state->P || !state->pot || !state->vnew ||
         !state->q || !state->eorb || !state->emo || !state->occ || !state->hist_x ||
         !state->hist_f || !state->qsh || !state->out || !state->order)

@@ -421,7 +421,10 @@ constexpr int JB = 16;
 constexpr int JS = 2 * JB;                  // subproblem dimension
 constexpr int JPK = JS * (JS + 1) / 2;      // packed subproblem
 constexpr int JWAVE_LDS = JPK + JS * JS + JS;  // M (packed), Q, c/s
-constexpr int NBLOCK_LDS = NWAVE * JWAVE_LDS + 64;  // + identity flags
+constexpr int NPAIR32 = JS / 2;                      // 16 pairs per round
+constexpr int NBLK32 = NPAIR32 * (NPAIR32 + 1) / 2;  // 136 blocks
+constexpr int SCHED_INTS = (JS - 1) * NPAIR32 + NBLK32;
+constexpr int NBLOCK_LDS = NWAVE * JWAVE_LDS + 64 + (SCHED_INTS + 1) / 2;  // + flags, schedules
 
 __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -448,15 +451,33 @@ __device__ inline void rr32(int r, int k, int& p, int& q) {
     else { p = (r + k) % 31; q = (r - k + 31) % 31; }
 }
 
+// Schedules of the 32-slot subproblem, built once per block-Jacobi call in
+// LDS: rrtab[r*16+k] = p | q<<8 for pair k of round r, and blktab[e] =
+// P | R<<8 for the e-th 2x2 block (P <= R) of a round.
+
+__device__ inline void build_schedules(int* tab) {
+    for (int e = threadIdx.x; e < (JS - 1) * NPAIR32; e += NT) {
+        int p, q;
+        rr32(e / NPAIR32, e % NPAIR32, p, q);
+        tab[e] = p | (q << 8);
+    }
+    for (int e = threadIdx.x; e < NBLK32; e += NT) {
+        int P = 0, rem = e;
+        while (rem >= NPAIR32 - P) { rem -= NPAIR32 - P; ++P; }
+        tab[(JS - 1) * NPAIR32 + e] = P | ((P + rem) << 8);
+    }
+}
+
 // Wave-level cyclic Jacobi on the packed JS x JS matrix M (slots whose
-// rows/columns are zero are never rotated); Q (row-major JS x JS) accumulates
-// the rotations so that M_final = Q^T M_0 Q.  Returns 1 if any rotation was
-// applied, 0 if M was already diagonal to `tol` (Q = I).
-__device__ __noinline__ int wave_jacobi_sub(double* M, double* Q, double* cs, double tol, int max_sweeps) {
-    constexpr int NPAIR = JS / 2;                   // 16 pairs per round
-    constexpr int NBLK = NPAIR * (NPAIR + 1) / 2;   // 136 2x2 blocks with P <= R
+// rows/columns are zero are never rotated).  Qt (JS x JS, Qt[col][row])
+// accumulates the rotations so that M_final = Q^T M_0 Q.  Returns 1 if any
+// rotation was applied, 0 if M was already diagonal to `tol` (Q = I).
+__device__ __noinline__ int wave_jacobi_sub(double* M, double* Qt, double* cs, const int* sched,
+                                            double tol, int max_sweeps) {
     const int lane = threadIdx.x % WAVE;
-    for (int e = lane; e < JS * JS; e += WAVE) Q[e] = ((e / JS) == (e % JS)) ? 1.0 : 0.0;
+    const int* rrtab = sched;
+    const int* blktab = sched + (JS - 1) * NPAIR32;
+    for (int e = lane; e < JS * JS; e += WAVE) Qt[e] = ((e / JS) == (e % JS)) ? 1.0 : 0.0;
     wave_sync();
     const double skip = 1e-3 * tol;
     int rotated = 0;
@@ -470,9 +491,10 @@ __device__ __noinline__ int wave_jacobi_sub(double* M, double* Q, double* cs, do
         if (off <= tol) break;
         rotated = 1;
         for (int r = 0; r < JS - 1; ++r) {
-            if (lane < NPAIR) {
-                int p, q;
-                rr32(r, lane, p, q);
+            const int* rr = rrtab + r * NPAIR32;
+            if (lane < NPAIR32) {
+                const int pq = rr[lane];
+                const int p = pq & 255, q = pq >> 8;
                 double c = 1.0, s = 0.0;
                 const double apq = pk(M, p, q);
                 if (fabs(apq) > skip) {
@@ -486,16 +508,13 @@ __device__ __noinline__ int wave_jacobi_sub(double* M, double* Q, double* cs, do
                 cs[JB + lane] = s;
             }
             wave_sync();
-            for (int e = lane; e < NBLK; e += WAVE) {
-                // triangular index e -> (P, R), P <= R
-                int P = 0, rem = e;
-                while (rem >= NPAIR - P) { rem -= NPAIR - P; ++P; }
-                const int R = P + rem;
+            for (int e = lane; e < NBLK32; e += WAVE) {
+                const int PR = blktab[e];
+                const int P = PR & 255, R = PR >> 8;
                 const double sp = cs[JB + P], sr = cs[JB + R];
                 if (sp == 0.0 && sr == 0.0) continue;
-                int p, q, x, y;
-                rr32(r, P, p, q);
-                rr32(r, R, x, y);
+                const int pq = rr[P], xy = rr[R];
+                const int p = pq & 255, q = pq >> 8, x = xy & 255, y = xy >> 8;
                 const double cp = cs[P], cr = cs[R];
                 if (P == R) {
                     const double apq = pk(M, p, q);
@@ -514,16 +533,17 @@ __device__ __noinline__ int wave_jacobi_sub(double* M, double* Q, double* cs, do
                     pk(M, q, y) = r10 * sr + r11 * cr;
                 }
             }
-            for (int e = lane; e < NPAIR * JS; e += WAVE) {
-                const int k = e / JS, row = e % JS;
+            // columns p, q of Q are rows p, q of Qt: lanes walk a row
+            const int row = lane % JS;
+            for (int k = lane / JS; k < NPAIR32; k += WAVE / JS) {
                 const double s = cs[JB + k];
                 if (s == 0.0) continue;
-                int p, q;
-                rr32(r, k, p, q);
+                const int pq = rr[k];
+                const int p = pq & 255, q = pq >> 8;
                 const double c = cs[k];
-                const double a = Q[row * JS + p], b = Q[row * JS + q];
-                Q[row * JS + p] = c * a - s * b;
-                Q[row * JS + q] = s * a + c * b;
+                const double a = Qt[p * JS + row], b = Qt[q * JS + row];
+                Qt[p * JS + row] = c * a - s * b;
+                Qt[q * JS + row] = s * a + c * b;
             }
             wave_sync();
         }
@@ -651,7 +671,7 @@ __device__ __noinline__ void rows_update(double* X, int n, size_t ld, int ncol, 
 // or -1 when max_sweeps were spent.
 template <class CAcc>
 __device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Qbuf, int max_sweeps, double tol,
-                                            double* lds, double* red, double* tacc) {
+                                            double* lds, double* red, double* tacc, int inner) {
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     double* M = lds + w * JWAVE_LDS;
     double* Q = M + JPK;
@@ -662,6 +682,8 @@ __device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Q
     const int ntile = npair * (npair - 1) / 2;
     const int nch = (n + WAVE - 1) / WAVE;
     int* ident = reinterpret_cast<int*>(lds + NWAVE * JWAVE_LDS);  // per pair: Q == I
+    int* sched = ident + 128;                                       // subproblem schedules
+    build_schedules(sched);
     // symmetrise from the upper triangle
     for (int e = threadIdx.x; e < n * n; e += NT) {
         const int i = e / n, j = e % n;
@@ -696,12 +718,12 @@ __device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Q
                     }
                 }
                 wave_sync();
-                const int rot = wave_jacobi_sub(M, Q, cs, tol, 2);
+                const int rot = wave_jacobi_sub(M, Q, cs, sched, tol, inner);
                 double* qg = Qbuf + (size_t)k * JS * JS;
                 if (lane == 0) ident[k] = rot ? 0 : 1;
                 if (rot) {
                     for (int e = lane; e < JS * JS; e += WAVE) {
-                        qg[e] = Q[e];
+                        qg[e] = Q[(e % JS) * JS + e / JS];  // Q[i][j] = Qt[j][i]
                         const int i = e / JS, j = e % JS;
                         const int gi = slot_index(I, J, i, n), gj = slot_index(I, J, j, n);
                         if (gi >= 0 && gj >= 0) A[(size_t)gi * n + gj] = pk(M, i, j);

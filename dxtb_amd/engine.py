@@ -189,13 +189,16 @@ class ScfOptions:
     diag_offset: float = 0.01
     jacobi_max_sweeps: int = 40
     jacobi_tol: float = 1e-13
+    jacobi_inner: int = 2
+    eig_inexact: float = 1e-2
 
     def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
         return _lib.XtbScfOpts(
             maxiter=self.maxiter, mix_guess=int(self.mix_guess), generations=self.generations,
             fermi_maxiter=self.fermi_maxiter, want_w=int(want_w),
-            jacobi_max_sweeps=self.jacobi_max_sweeps, padded=int(padded), pad0=0,
+            jacobi_max_sweeps=self.jacobi_max_sweeps, padded=int(padded),
+            jacobi_inner=self.jacobi_inner, eig_inexact=self.eig_inexact,
             x_atol=self.x_atol, x_atol_max=self.x_atol_max, kt=self.fermi_etemp * C.KELVIN2AU,
             damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
             fermi_thresh=thr, jacobi_tol=self.jacobi_tol,

@@ -89,6 +89,8 @@ extern "C" {
 #define XTB_OUT_T_BSUB 15   /* block Jacobi: subproblem solves */
 #define XTB_OUT_T_BROW 16   /* block Jacobi: row-panel updates (A and vectors) */
 #define XTB_OUT_T_BCOL 17   /* block Jacobi: column-panel updates */
+#define XTB_OUT_SWEEPS_FIRST 18  /* Jacobi sweeps of the first SCF step */
+#define XTB_OUT_SWEEPS_LAST 19   /* Jacobi sweeps of the final (exact) step */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {
@@ -165,7 +167,7 @@ typedef struct xtb_scf_opts {
     int32_t want_w;          /* also form the energy-weighted density */
     int32_t jacobi_max_sweeps;
     int32_t padded;          /* 1: padded layout semantics */
-    int32_t pad0;
+    int32_t jacobi_inner;    /* sweeps per 32x32 block-Jacobi subproblem */
     double x_atol;
     double x_atol_max;
     double kt;               /* electronic temperature, Hartree */
@@ -174,6 +176,7 @@ typedef struct xtb_scf_opts {
     double diag_offset;
     double fermi_thresh;
     double jacobi_tol;
+    double eig_inexact;      /* eigensolve tolerance = this x last |dv|_inf (0: exact) */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */
