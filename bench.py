@@ -64,6 +64,62 @@ def cpu_baseline(stream, seconds: float) -> dict:
                       f"{os.cpu_count()} logical CPUs"}
 
 
+def forces_cost(reqs, device, opts, repeats: int = 3) -> dict:
+    """Wall time of one ragged batch for energy only and for energy+forces
+    (median of `repeats`, after one warm-up of each)."""
+    import torch
+
+    from dxtb_amd.engine import BatchLayout, DeviceTables, Pipeline
+    from dxtb_amd.param import SpeciesTables
+
+    mols = [r.numbers for r in reqs]
+    tb = SpeciesTables(np.unique(np.concatenate(mols)))
+    pipe = Pipeline(BatchLayout(mols, tb, padded=False), DeviceTables(tb, device), device)
+    pos = torch.as_tensor(np.concatenate([r.positions for r in reqs]), dtype=torch.float64,
+                          device=device).contiguous()
+    out = {}
+    for forces in (False, True):
+        ts = []
+        for k in range(repeats + 1):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            pipe.run(pos, opts, forces=forces)
+            torch.cuda.synchronize()
+            ts.append(time.perf_counter() - t0)
+        out["forces" if forces else "energy"] = float(np.median(ts[1:]))
+    return {"energy_ms": round(out["energy"] * 1e3, 2), "energy_forces_ms": round(out["forces"] * 1e3, 2),
+            "ratio": round(out["forces"] / out["energy"], 3), "molecules": len(reqs)}
+
+
+def device_peaks(device) -> dict:
+    """Achievable FP64 GEMM rate (torch.matmul, n=8192) and device copy
+    bandwidth (1 GiB), for context next to the published peaks."""
+    import torch
+
+    a = torch.randn(8192, 8192, dtype=torch.float64, device=device)
+    b = torch.randn_like(a)
+    for _ in range(2):
+        a @ b
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        a @ b
+    torch.cuda.synchronize()
+    gemm = 3 * 2 * 8192.0**3 / (time.perf_counter() - t0) / 1e12
+    del a, b
+    x = torch.empty(2**27, dtype=torch.float64, device=device)
+    y = torch.empty_like(x)
+    for _ in range(2):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    bw = 5 * 2 * x.numel() * 8 / (time.perf_counter() - t0) / 1e9
+    return {"dgemm_tflops": round(gemm, 2), "copy_gbs": round(bw, 1)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +216,11 @@ def main() -> int:
         padded = padded_dropin(pr, a.padded_batch, dev, opts=opts)["molecules_per_s"]
         log(f"padded drop-in: {padded:.1f} molecules/s")
 
+    fc = forces_cost(_requests(stream, a.warmup * a.chunk, min(a.chunk, 128)), dev, opts)
+    log(f"forces cost: {fc}")
+    peaks = device_peaks(dev)
+    log(f"measured peaks: {peaks}")
+
     cpu = None if a.no_cpu else cpu_baseline(stream, a.cpu_seconds)
     if cpu:
         log(f"cpu baseline: {cpu['value']:.3f} molecules/s")
@@ -178,11 +239,13 @@ def main() -> int:
             "padded_batch": a.padded_batch,
             "latency_rate_per_s": round(rate, 2), "p50_ms": round(lat["p50_ms"], 2),
             "p99_ms": round(lat["p99_ms"], 2), "nonconverged": bad,
+            "forces_over_energy": fc,
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
                      "traffic": None, "kernel": "xtb::k_scf",
-                     "launches": int(len(ms)), "mean_launch_ms": round(float(ms.mean()), 3)},
+                     "launches": int(len(ms)), "mean_launch_ms": round(float(ms.mean()), 3),
+                     "measured_peaks": peaks},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

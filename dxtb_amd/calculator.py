@@ -43,6 +43,9 @@ from .exceptions import (
 from .param import SpeciesTables
 
 _DISP_WARNED = False
+# single molecules with more orbitals than this run on the whole-GPU path
+# (dxtb_amd.large); batches use the per-molecule kernel (limit 1024)
+LARGE_N = 600
 
 
 def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
@@ -189,6 +192,11 @@ class GFN1Calculator:
             return hit[1]
         if hit is not None and forces is False and hit[0][:3] == ckey[:3] and hit[0][4:] == ckey[4:]:
             return hit[1]
+        if not self.batched:
+            from .serving import orbital_count
+
+            if orbital_count(self._mols[0]) > LARGE_N:
+                return self._run_large(positions, chrg, spin, forces, ckey)
         pipe = self._pipeline(chrg, spin)
         pos = positions.detach()
         flat = pos[self._mask].contiguous() if self.batched else pos[self._mask].contiguous()
@@ -210,6 +218,37 @@ class GFN1Calculator:
         if forces:
             g = torch.zeros(positions.shape, dtype=torch.float64, device=self.device)
             g[self._mask] = pipe.gradient().clone()
+            res["gradient_full"] = g
+        self._cache["run"] = (ckey, res)
+        return res
+
+    def _run_large(self, positions, chrg, spin, forces, ckey) -> dict:
+        from .large import LargeSystem
+
+        if spin is not None:
+            raise NotImplementedError("spin for the large-system path")
+        c = float(self._charges_array(chrg)[0])
+        if self._pipe_key != ("large", c):
+            self._pipe = LargeSystem(self._mols[0], self.device, opts=self.scf_opts, chrg=c)
+            self._pipe_key = ("large", c)
+        pos = positions.detach()[self._mask].contiguous()
+        r = self._pipe.run(pos, forces=forces)
+        if not r["converged"]:
+            msg = f"SCF does not converge after {self.scf_opts.maxiter} cycles."
+            if self.force_convergence:
+                raise SCFConvergenceError(msg)
+            warnings.warn(msg, SCFConvergenceWarning)
+        e = torch.tensor(r["energy"], dtype=torch.float64, device=self.device)
+        if {"rep"} & self.exclude:
+            e = e - r["repulsion"]
+        rec = torch.zeros((1, _lib.OUT_STRIDE), dtype=torch.float64, device=self.device)
+        rec[0, _lib.OUT_ESCF] = r["scf"]
+        rec[0, _lib.OUT_EREP] = r["repulsion"]
+        rec[0, _lib.OUT_ITER] = r["iterations"]
+        res = {"energy": e, "records": rec, "pipeline": self._pipe.pipe, "large": r}
+        if forces:
+            g = torch.zeros(positions.shape, dtype=torch.float64, device=self.device)
+            g[self._mask] = r["gradient"]
             res["gradient_full"] = g
         self._cache["run"] = (ckey, res)
         return res
@@ -268,6 +307,8 @@ class GFN1Calculator:
     def get_charges(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> torch.Tensor:
         """Orbital-resolved Mulliken charges, shape (..., nao) (types/abc.py:429-486)."""
         res = self._run(positions, chrg, spin, forces=False)
+        if "large" in res:
+            return res["large"]["charges_orbital"].clone()
         pipe = res["pipeline"]
         L = pipe.L
         nmax = int(L.nao.max())
@@ -281,6 +322,9 @@ class GFN1Calculator:
 
     def get_mo_energies(self, positions, chrg=0, spin=None):
         res = self._run(positions, chrg, spin, forces=False)
+        if "large" in res:
+            raise PropertyNotImplementedError(
+                "orbital energies: the large-system path builds the density by purification")
         pipe = res["pipeline"]
         L = pipe.L
         out = [pipe.emo[int(L.vec0[b]): int(L.vec0[b]) + int(L.n[b])].clone() for b in range(L.B)]

@@ -11,7 +11,7 @@
 //   mixed with upstream's Anderson mixer (scf/mixer/anderson.py:163-317).
 //
 // Eigensolver: the generalized problem is reduced once per geometry with the
-// Loewdin orthogonaliser X = S^-1/2 (itself from a Jacobi solve of S).  Every
+// Loewdin orthogonaliser X = S^-1/2 (coupled Newton-Schulz on matrix cores).  Every
 // SCF step transforms F into the basis of the previous step's orbitals,
 // A = C^T F C, which is close to diagonal, and finishes it with a few sweeps
 // of parallel-ordered cyclic Jacobi whose rotations are applied directly to
