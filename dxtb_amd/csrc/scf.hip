@@ -38,7 +38,7 @@ __host__ __device__ inline Lay make_lay(int nmax, int max_nsh, int max_nat) {
     const int npow = next_pow2(nmax < 2 ? 2 : nmax);
     int o = 0;
     L.red = o; o += 256;
-    L.cs = o; o += 2 * (nmax / 2 + 2);
+    L.cs = o; o += 3 * (nmax / 2 + 2);
     L.skey = o; o += npow;
     L.sidx = o; o += (npow + 1) / 2;  // ints
     L.ev = o; o += nmax;
@@ -56,16 +56,16 @@ __host__ __device__ inline Lay make_lay(int nmax, int max_nsh, int max_nat) {
 struct Mol {
     int m, n, nao, nsh, nat, vec0, sh0, at0;
     size_t mat0, gam0;
-    const double *S, *H0, *gam;
-    double *F, *Ct, *T, *P;
-    double *pot, *vnew, *q, *eorb, *emo, *occ, *qsh_out;
+    const glbf *S, *H0, *gam;
+    glbf *F, *Ct, *T, *P;
+    glbf *pot, *vnew, *q, *eorb, *emo, *occ, *qsh_out;
     double nel_a, nel_b;
     int nvec_total;
 };
 
 struct Shared {
-    double *red, *cs, *skey, *ev, *ftot, *qsh, *vsh, *gq, *qat, *big;
-    int* sidx;
+    ldsf *red, *cs, *skey, *ev, *ftot, *qsh, *vsh, *gq, *qat, *big;
+    ldsi* sidx;
     int npow, big_cap;
     double* tacc;  // per-thread stage timers (meaningful on thread 0)
 };
@@ -89,12 +89,10 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
     int sweeps;
     if (npk <= sh.big_cap) {
         SymPacked A{sh.big, n};
-        for (int idx = threadIdx.x; idx < n * n; idx += NT) {
-            const int i = idx / n, j = idx % n;
-            if (j >= i) A.at(i, j) = M.F[(size_t)i * n + j];
-        }
+        for (int i = threadIdx.x / WAVE; i < n; i += NWAVE)
+            for (int j = i + threadIdx.x % WAVE; j < n; j += WAVE) A.at(i, j) = M.F[(size_t)i * n + j];
         const bool cl = (npk + n * n) <= sh.big_cap;
-        double* cbuf = sh.big + npk;
+        ldsf* cbuf = sh.big + npk;
         if (cl) {
             for (int idx = threadIdx.x; idx < n * n; idx += NT) cbuf[idx] = M.Ct[idx];
         }
@@ -134,27 +132,30 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
         rs = fmax(rs, a);
     }
     const double c = wg_max(rs, sh.red) * 1.0001;
-    double* Y = M.F;
-    double* Z = M.Ct;
-    double* T = M.T;
-    double* W = M.P;
-    for (size_t e = threadIdx.x; e < nn; e += NT) {
-        const int r = (int)(e / n), q = (int)(e % n);
-        Y[e] = M.S[e] / c;
-        Z[e] = (r == q) ? 1.0 : 0.0;
-    }
+    glbf* Y = M.F;
+    glbf* Z = M.Ct;
+    glbf* T = M.T;
+    glbf* W = M.P;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int r = w; r < n; r += NWAVE)
+        for (int q = lane; q < n; q += WAVE) {
+            const size_t e = (size_t)r * n + q;
+            Y[e] = M.S[e] / c;
+            Z[e] = (r == q) ? 1.0 : 0.0;
+        }
     __syncthreads();
     int it = 0;
     int extra = -1;  // iterations left once in the quadratic regime
     for (; it < 60; ++it) {
         gemm<false, false>(n, n, n, -0.5, Z, n, Y, n, 0.0, T, n, sh.big);  // -(Z Y)/2
         double err = 0.0;
-        for (size_t e = threadIdx.x; e < nn; e += NT) {
-            const int r = (int)(e / n), q = (int)(e % n);
-            const double v = T[e] + ((r == q) ? 1.5 : 0.0);
-            err = fmax(err, fabs(v - ((r == q) ? 1.0 : 0.0)));
-            T[e] = v;
-        }
+        for (int r = w; r < n; r += NWAVE)
+            for (int q = lane; q < n; q += WAVE) {
+                const size_t e = (size_t)r * n + q;
+                const double v = T[e] + ((r == q) ? 1.5 : 0.0);
+                err = fmax(err, fabs(v - ((r == q) ? 1.0 : 0.0)));
+                T[e] = v;
+            }
         err = wg_max(err, sh.red);
         if (extra == 0 || err < 1e-15) break;
         if (extra > 0) --extra;
@@ -162,7 +163,7 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
         gemm<false, false>(n, n, n, 1.0, Y, n, T, n, 0.0, W, n, sh.big);  // Y T
         gemm<false, false>(n, n, n, 1.0, T, n, Z, n, 0.0, Y, n, sh.big);  // T Z (into Y's buffer)
         // rotate buffers: Y <- W, Z <- (T Z) held in Y's buffer
-        double* oldZ = Z;
+        glbf* oldZ = Z;
         Z = Y;
         Y = W;
         W = oldZ;
@@ -172,14 +173,13 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
     for (size_t e = threadIdx.x; e < nn; e += NT) M.Ct[e] = Z[e] * sc;  // Z may alias Ct
     __syncthreads();
     // symmetrise (rounding)
-    for (size_t e = threadIdx.x; e < nn; e += NT) {
-        const int r = (int)(e / n), q = (int)(e % n);
-        if (q > r) {
+    for (int r = w; r < n; r += NWAVE)
+        for (int q = r + 1 + lane; q < n; q += WAVE) {
+            const size_t e = (size_t)r * n + q;
             const double a = 0.5 * (M.Ct[e] + M.Ct[(size_t)q * n + r]);
             M.Ct[e] = a;
             M.Ct[(size_t)q * n + r] = a;
         }
-    }
     __syncthreads();
     return 0;
 }
@@ -191,7 +191,7 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
 // Writes the sorted occupations to occ_out and adds them to sh.ftot.
 // ---------------------------------------------------------------------------
 __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, double nel, double kt, double thr,
-                              int maxiter, bool padded, double* occ_out, double* efermi) {
+                              int maxiter, bool padded, glbf* occ_out, double* efermi) {
     const int n = M.n;
     const double eps = 2.220446049250313e-16;
     if (fabs(nel) < eps) {
@@ -246,14 +246,18 @@ __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, doubl
 // leaves P, q, e_orb, sorted e/occupations and shell/atom charges behind.
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
-                        const xtb_params& par, const xtb_scf_opts& o, const double* vin,
-                        double* vout, int* status, double* ef, double loose) {
+                        const xtb_params& par, const xtb_scf_opts& o, const glbf* vin,
+                        glbf* vout, int* status, double* ef, double loose) {
     const int n = M.n;
     const size_t nn = (size_t)n * n;
     // Fock build (scf/base.py:817-859)
-    for (size_t e = threadIdx.x; e < nn; e += NT) {
-        const int r = (int)(e / n), c = (int)(e % n);
-        M.F[e] = M.H0[e] - 0.5 * M.S[e] * (vin[r] + vin[c]);
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int r = w; r < n; r += NWAVE) {
+        const double vr = vin[r];
+        for (int c = lane; c < n; c += WAVE) {
+            const size_t e = (size_t)r * n + c;
+            M.F[e] = M.H0[e] - 0.5 * M.S[e] * (vr + vin[c]);
+        }
     }
     __syncthreads();
     // T = F C ; A = C^T T  -> F
@@ -281,7 +285,10 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
                        M.occ + M.nvec_total, &ef[1]) && ok;
     if (!ok) *status |= XTB_MOL_FERMI;
     // P = C diag(f) C^T
-    for (size_t e = threadIdx.x; e < nn; e += NT) M.T[e] = sh.ftot[e / n] * M.Ct[e];
+    for (int r = w; r < n; r += NWAVE) {
+        const double f = sh.ftot[r];
+        for (int c = lane; c < n; c += WAVE) M.T[(size_t)r * n + c] = f * M.Ct[(size_t)r * n + c];
+    }
     __syncthreads();
     t0 = now_us();
     gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.P, n, sh.big);
@@ -292,9 +299,9 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         for (int i = w; i < n; i += NWAVE) {
             double sq = 0.0, se = 0.0;
             if (i < M.nao) {
-                const double* pr = M.P + (size_t)i * n;
-                const double* sr = M.S + (size_t)i * n;
-                const double* hr = M.H0 + (size_t)i * n;
+                const glbf* pr = M.P + (size_t)i * n;
+                const glbf* sr = M.S + (size_t)i * n;
+                const glbf* hr = M.H0 + (size_t)i * n;
                 for (int j = l; j < M.nao; j += WAVE) {
                     sq = fma(pr[j], sr[j], sq);
                     se = fma(pr[j], hr[j], se);
@@ -329,7 +336,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     {
         const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
         for (int s = w; s < M.nsh; s += NWAVE) {
-            const double* gr = M.gam + (size_t)s * M.nsh;
+            const glbf* gr = M.gam + (size_t)s * M.nsh;
             double acc = 0.0;
             for (int t = l; t < M.nsh; t += WAVE) acc = fma(gr[t], sh.qsh[t], acc);
             acc = wave_sum(acc);
@@ -360,20 +367,20 @@ struct Mixer {
 };
 
 __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh, const xtb_scf_opts& o,
-                         double* hx, double* hf, const double* xnew, const double* xold,
-                         double* xout, double* resid_inf) {
+                         glbf* hx, glbf* hf, const glbf* xnew, const glbf* xold,
+                         glbf* xout, double* resid_inf) {
     const int n = M.n, G = mx.G;
     auto X = [&](int h) { return hx + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
     auto Fh = [&](int h) { return hf + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
     if (mx.step == 0) {
-        double* x0 = X(0);
+        glbf* x0 = X(0);
         for (int i = threadIdx.x; i < n; i += NT) x0[i] = xold[i];
         __syncthreads();
     }
     mx.step += 1;
     const int gen = G - 1;
-    double* x0 = X(0);
-    double* f0 = Fh(0);
+    glbf* x0 = X(0);
+    glbf* f0 = Fh(0);
     double l2 = 0.0, linf = 0.0;
     for (int i = threadIdx.x; i < n; i += NT) {
         const double f = xnew[i] - x0[i];
@@ -409,8 +416,8 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
         wg_sum_n<20>(acc, sh.red);
         // solve (a * diag(1 + off^2)) theta = b by Gaussian elimination with
         // partial pivoting; one thread, system held in LDS
-        double* a = sh.red;          // [5][6]
-        double* th = sh.red + 32;    // [5]
+        ldsf* a = sh.red;          // [5][6]
+        ldsf* th = sh.red + 32;    // [5]
         if (threadIdx.x == 0) {
             int k = 0;
 #pragma unroll
@@ -448,7 +455,7 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
     }
     // new slot for the mixed vector = oldest slot
     const int newhead = (mx.head - 1 + G) % G;
-    double* xn = hx + (size_t)newhead * M.nvec_total + M.vec0;
+    glbf* xn = hx + (size_t)newhead * M.nvec_total + M.vec0;
     for (int i = threadIdx.x; i < n; i += NT) {
         double xm;
         if (nn > 0) {
@@ -479,17 +486,18 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     extern __shared__ double lds[];
     const Lay L = make_lay(b.max_n, b.max_nsh, b.max_nat);
     Shared sh;
-    sh.red = lds + L.red;
-    sh.cs = lds + L.cs;
-    sh.skey = lds + L.skey;
-    sh.sidx = reinterpret_cast<int*>(lds + L.sidx);
-    sh.ev = lds + L.ev;
-    sh.ftot = lds + L.ftot;
-    sh.qsh = lds + L.qsh;
-    sh.vsh = lds + L.vsh;
-    sh.gq = lds + L.gq;
-    sh.qat = lds + L.qat;
-    sh.big = lds + L.big;
+    ldsf* ls = to_lds(lds);
+    sh.red = ls + L.red;
+    sh.cs = ls + L.cs;
+    sh.skey = ls + L.skey;
+    sh.sidx = reinterpret_cast<ldsi*>(ls + L.sidx);
+    sh.ev = ls + L.ev;
+    sh.ftot = ls + L.ftot;
+    sh.qsh = ls + L.qsh;
+    sh.vsh = ls + L.vsh;
+    sh.gq = ls + L.gq;
+    sh.qat = ls + L.qat;
+    sh.big = ls + L.big;
     sh.npow = next_pow2(b.max_n < 2 ? 2 : b.max_n);
     sh.big_cap = big_cap;
 
@@ -504,20 +512,20 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     M.at0 = b.mol_atom0[M.m];
     M.mat0 = (size_t)b.mol_mat0[M.m];
     M.gam0 = (size_t)b.mol_gam0[M.m];
-    M.S = S + M.mat0;
-    M.H0 = H0 + M.mat0;
-    M.gam = gam + M.gam0;
-    M.F = st.F + M.mat0;
-    M.Ct = st.C + M.mat0;
-    M.T = st.T + M.mat0;
-    M.P = st.P + M.mat0;
-    M.pot = st.pot + M.vec0;
-    M.vnew = st.vnew + M.vec0;
-    M.q = st.q + M.vec0;
-    M.eorb = st.eorb + M.vec0;
-    M.emo = st.emo + M.vec0;
-    M.occ = st.occ + M.vec0;
-    M.qsh_out = st.qsh + M.sh0;
+    M.S = to_glb(S) + M.mat0;
+    M.H0 = to_glb(H0) + M.mat0;
+    M.gam = to_glb(gam) + M.gam0;
+    M.F = to_glb(st.F) + M.mat0;
+    M.Ct = to_glb(st.C) + M.mat0;
+    M.T = to_glb(st.T) + M.mat0;
+    M.P = to_glb(st.P) + M.mat0;
+    M.pot = to_glb(st.pot) + M.vec0;
+    M.vnew = to_glb(st.vnew) + M.vec0;
+    M.q = to_glb(st.q) + M.vec0;
+    M.eorb = to_glb(st.eorb) + M.vec0;
+    M.emo = to_glb(st.emo) + M.vec0;
+    M.occ = to_glb(st.occ) + M.vec0;
+    M.qsh_out = to_glb(st.qsh) + M.sh0;
     M.nel_a = b.mol_nel[2 * M.m];
     M.nel_b = b.mol_nel[2 * M.m + 1];
     M.nvec_total = b.nvec_total;
@@ -549,7 +557,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     sweeps += sw_first;
     ++iters;
     if (o.mix_guess) {
-        anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot, &resid);
+        anderson(mx, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, &resid);
     } else {
         for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = M.vnew[i];
         __syncthreads();
@@ -558,7 +566,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     for (int it = 0; it < o.maxiter; ++it) {
         sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, o.eig_inexact * resid);
         ++iters;
-        const bool c = anderson(mx, M, sh, o, st.hist_x, st.hist_f, M.vnew, M.pot, M.pot, &resid);
+        const bool c = anderson(mx, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, &resid);
         if (c) { converged = true; break; }
     }
     if (!converged && o.maxiter > 0) status |= XTB_MOL_MAXITER;
@@ -592,10 +600,10 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     for (int j = threadIdx.x; j < M.n; j += NT) M.emo[j] = sh.skey[j];
     if (o.want_w) {
         // W = C diag(f e) C^T (analytical.py:186-190)
-        const size_t nn = (size_t)M.n * M.n;
-        for (size_t e = threadIdx.x; e < nn; e += NT) {
-            const int k = (int)(e / M.n);
-            M.F[e] = sh.ftot[k] * sh.ev[k] * M.Ct[e];
+        for (int k = threadIdx.x / WAVE; k < M.n; k += NWAVE) {
+            const double fe = sh.ftot[k] * sh.ev[k];
+            for (int c = threadIdx.x % WAVE; c < M.n; c += WAVE)
+                M.F[(size_t)k * M.n + c] = fe * M.Ct[(size_t)k * M.n + c];
         }
         __syncthreads();
         gemm<true, false>(M.n, M.n, M.n, 1.0, M.Ct, M.n, M.F, M.n, 0.0, M.T, M.n, sh.big);

@@ -16,6 +16,18 @@ constexpr int NT = XTB_NT;
 constexpr int WAVE = 64;
 constexpr int NWAVE = NT / WAVE;
 
+// Address-space-qualified pointers.  A function compiled out of line cannot
+// tell whether a plain pointer addresses LDS or global memory and falls back
+// to flat instructions (slower, and every wait covers both kinds of access);
+// these types keep ds_* and global_* addressing across calls.
+typedef __attribute__((address_space(3))) double ldsf;
+typedef __attribute__((address_space(3))) int ldsi;
+typedef __attribute__((address_space(1))) double glbf;
+__device__ inline ldsf* to_lds(double* p) { return (ldsf*)p; }
+__device__ inline ldsi* to_lds(int* p) { return (ldsi*)p; }
+__device__ inline glbf* to_glb(double* p) { return (glbf*)p; }
+__device__ inline const glbf* to_glb(const double* p) { return (const glbf*)p; }
+
 __device__ inline double wave_sum(double v) {
 #pragma unroll
     for (int o = WAVE / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
@@ -30,7 +42,7 @@ __device__ inline double wave_max(double v) {
 
 // Sum over the workgroup; `red` holds >= NWAVE doubles of LDS.  Result is
 // returned to every thread.
-__device__ inline double wg_sum(double v, double* red) {
+__device__ inline double wg_sum(double v, ldsf* red) {
     v = wave_sum(v);
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     __syncthreads();
@@ -43,7 +55,7 @@ __device__ inline double wg_sum(double v, double* red) {
     return s;
 }
 
-__device__ inline double wg_max(double v, double* red) {
+__device__ inline double wg_max(double v, ldsf* red) {
     v = wave_max(v);
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
     __syncthreads();
@@ -59,7 +71,7 @@ __device__ inline double wg_max(double v, double* red) {
 // Several sums at once: vals[K] per thread -> out[K] on every thread.
 // `red` must hold K*NWAVE doubles.
 template <int K>
-__device__ inline void wg_sum_n(double (&vals)[K], double* red) {
+__device__ inline void wg_sum_n(double (&vals)[K], ldsf* red) {
     const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
 #pragma unroll
     for (int k = 0; k < K; ++k) vals[k] = wave_sum(vals[k]);
@@ -74,97 +86,6 @@ __device__ inline void wg_sum_n(double (&vals)[K], double* red) {
         double s = 0.0;
         for (int i = 0; i < NWAVE; ++i) s += red[k * NWAVE + i];
         vals[k] = s;
-    }
-    __syncthreads();
-}
-
-// ---------------------------------------------------------------------------
-// GEMM: C = alpha * op(A) * op(B) + beta * C, all row-major in global memory.
-// op(A)[m][k] = TA ? A[k*lda+m] : A[m*lda+k];  op(B)[k][n] likewise.
-// The workgroup splits into NT/256 groups; each group computes 64x64 output
-// tiles with a 4x4 register micro-tile per thread, staging 64x16 / 16x64
-// operand panels in LDS.  `lds` must hold NGEMM_LDS doubles.
-// ---------------------------------------------------------------------------
-constexpr int GBM = 64, GBN = 64, GBK = 16;
-constexpr int GPAD = GBM + 1;
-constexpr int NGROUP = NT / 256;
-constexpr int NGEMM_LDS = NGROUP * 2 * GBK * GPAD;
-
-template <bool TA, bool TB>
-__device__ __noinline__ void wg_gemm(int M, int N, int K, double alpha, const double* __restrict__ A, int lda,
-                        const double* __restrict__ B, int ldb, double beta, double* __restrict__ C,
-                        int ldc, double* lds) {
-    const int g = threadIdx.x / 256;
-    const int t = threadIdx.x % 256;
-    double* As = lds + g * 2 * GBK * GPAD;  // [GBK][GPAD]
-    double* Bs = As + GBK * GPAD;           // [GBK][GPAD]
-    const int tr = t / 16, tc = t % 16;
-    const int tm = (M + GBM - 1) / GBM, tn = (N + GBN - 1) / GBN;
-    const int ntiles = tm * tn;
-    // all groups run the same number of tile iterations (barriers inside)
-    const int iters = (ntiles + NGROUP - 1) / NGROUP;
-    for (int it = 0; it < iters; ++it) {
-        const int tile = it * NGROUP + g;
-        const bool active = tile < ntiles;
-        const int m0 = active ? (tile / tn) * GBM : 0;
-        const int n0 = active ? (tile % tn) * GBN : 0;
-        double acc[4][4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i)
-#pragma unroll
-            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
-        for (int k0 = 0; k0 < K; k0 += GBK) {
-            // stage op(A)[m0:m0+64][k0:k0+16] as As[k][m]
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int m, k;
-                if (TA) { m = t % 64; k = t / 64 + 4 * r; }
-                else { k = t % 16; m = t / 16 + 16 * r; }
-                const int gm = m0 + m, gk = k0 + k;
-                double val = 0.0;
-                if (active && gm < M && gk < K) val = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-                As[k * GPAD + m] = val;
-            }
-            // stage op(B)[k0:k0+16][n0:n0+64] as Bs[k][n]
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                int n, k;
-                if (TB) { k = t % 16; n = t / 16 + 16 * r; }
-                else { n = t % 64; k = t / 64 + 4 * r; }
-                const int gn = n0 + n, gk = k0 + k;
-                double val = 0.0;
-                if (active && gn < N && gk < K) val = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-                Bs[k * GPAD + n] = val;
-            }
-            __syncthreads();
-#pragma unroll
-            for (int k = 0; k < GBK; ++k) {
-                double a[4], b[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) a[i] = As[k * GPAD + tr * 4 + i];
-#pragma unroll
-                for (int j = 0; j < 4; ++j) b[j] = Bs[k * GPAD + tc * 4 + j];
-#pragma unroll
-                for (int i = 0; i < 4; ++i)
-#pragma unroll
-                    for (int j = 0; j < 4; ++j) acc[i][j] = fma(a[i], b[j], acc[i][j]);
-            }
-            __syncthreads();
-        }
-        if (active) {
-#pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int gm = m0 + tr * 4 + i;
-                if (gm >= M) continue;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int gn = n0 + tc * 4 + j;
-                    if (gn >= N) continue;
-                    double* c = C + (size_t)gm * ldc + gn;
-                    *c = (beta == 0.0) ? alpha * acc[i][j] : fma(alpha, acc[i][j], beta * *c);
-                }
-            }
-        }
     }
     __syncthreads();
 }
@@ -191,15 +112,15 @@ __device__ inline v4d v4zero() {
 // (TILE = 64 or 128), 8 waves as 2 (rows) x 4 (cols), k panels of 16 staged in
 // LDS as As[k][m], Bs[k][n].  `lds` holds >= 2*16*(TILE+4) doubles.
 template <bool TA, bool TB, int TILE>
-__device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const double* __restrict__ A,
-                                          int lda, const double* __restrict__ B, int ldb, double beta,
-                                          double* __restrict__ C, int ldc, double* lds) {
+__device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
+                                          int lda, const glbf* __restrict__ B, int ldb, double beta,
+                                          glbf* __restrict__ C, int ldc, ldsf* lds) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
     constexpr int LDT = TILE + 4;
     constexpr int E = TILE * 16 / NT;  // staged elements per thread and operand
     constexpr int FM = TILE / 32, FN = TILE / 64;
-    double* As = lds;
-    double* Bs = lds + 16 * LDT;
+    ldsf* As = lds;
+    ldsf* Bs = lds + 16 * LDT;
     const int t = threadIdx.x;
     const int w = t / WAVE, lane = t % WAVE, li = lane % 16, lg = lane / 16;
     const int wm = w / 4, wn = w % 4;
@@ -256,7 +177,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                     const int gm = m0 + wm * (TILE / 2) + i * 16 + lg + 4 * r;
                     const int gn = n0 + wn * (TILE / 4) + j * 16 + li;
                     if (gm < M && gn < N) {
-                        double* c = C + (size_t)gm * ldc + gn;
+                        glbf* c = C + (size_t)gm * ldc + gn;
                         *c = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * *c);
                     }
                 }
@@ -268,8 +189,8 @@ constexpr int NGEMM_MFMA_LDS = 2 * 16 * (128 + 4);
 
 // dispatch: matrix-core GEMM with a tile matched to the size
 template <bool TA, bool TB>
-__device__ inline void gemm(int M, int N, int K, double alpha, const double* A, int lda,
-                            const double* B, int ldb, double beta, double* C, int ldc, double* lds) {
+__device__ inline void gemm(int M, int N, int K, double alpha, const glbf* A, int lda,
+                            const glbf* B, int ldb, double beta, glbf* C, int ldc, ldsf* lds) {
     if (M <= 64 && N <= 64) wg_gemm_mfma<TA, TB, 64>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
     else wg_gemm_mfma<TA, TB, 128>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
 }
@@ -280,18 +201,18 @@ __device__ inline void gemm(int M, int N, int K, double alpha, const double* A, 
 // global memory.
 // ---------------------------------------------------------------------------
 struct SymPacked {
-    double* a;
+    ldsf* a;
     int n;
-    __device__ inline double& at(int i, int j) const {
+    __device__ inline ldsf& at(int i, int j) const {
         if (i > j) { int t = i; i = j; j = t; }
         return a[i * n - (i * (i - 1)) / 2 + (j - i)];
     }
 };
 
 struct SymFull {
-    double* a;
+    glbf* a;
     int n;
-    __device__ inline double& at(int i, int j) const {
+    __device__ inline glbf& at(int i, int j) const {
         if (i > j) { int t = i; i = j; j = t; }
         return a[(size_t)i * n + j];
     }
@@ -310,23 +231,25 @@ __device__ inline void rr_pair(int r, int k, int n2, int& p, int& q) {
 // Diagonalises the symmetric n x n matrix `A` in place (eigenvalues end on
 // the diagonal) and applies every rotation to the rows of Ct (Ct[k][mu],
 // row k = vector k), i.e. C <- C J.  Ct rows have stride `ldc`.
-// `cs` is LDS for 2*(n/2+1) doubles.  Returns the number of sweeps, or
-// -1 when max_sweeps were spent without reaching `tol` (absolute).
+// `cs` is LDS for 3*(n/2+1) doubles: cos, sin and the packed pair (p, q) of
+// each rotation of the round.  Loops are laid out wave x lane so that no
+// index needs an integer division.  Returns the number of sweeps, or -1
+// when max_sweeps were spent without reaching `tol` (absolute).
 template <class Sym, class CAcc>
-__device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, double* cs,
-                         double* red) {
+__device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf* cs,
+                         ldsf* red) {
     const int n2 = n + (n & 1);
     const int npair = n2 / 2;
-    double* cc = cs;
-    double* ss = cs + npair;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    ldsf* cc = cs;
+    ldsf* ss = cs + npair;
+    ldsi* pqs = reinterpret_cast<ldsi*>(cs + 2 * npair);  // p | q << 16
     int sweep = 0;
     for (;;) {
         // convergence test: largest off-diagonal element
         double off = 0.0;
-        for (int idx = threadIdx.x; idx < n * n; idx += NT) {
-            const int i = idx / n, j = idx % n;
-            if (j > i) off = fmax(off, fabs(A.at(i, j)));
-        }
+        for (int i = w; i < n; i += NWAVE)
+            for (int j = i + 1 + lane; j < n; j += WAVE) off = fmax(off, fabs(A.at(i, j)));
         off = wg_max(off, red);
         if (off <= tol) return sweep;
         if (sweep >= max_sweeps) return -1;
@@ -350,54 +273,57 @@ __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, doub
                 }
                 cc[k] = c;
                 ss[k] = s;
+                pqs[k] = p | (q << 16);
             }
             __syncthreads();
-            // 2a. A <- J^T A J on 2x2 blocks (P <= R)
-            const int nblk = npair * npair;
-            for (int idx = threadIdx.x; idx < nblk; idx += NT) {
-                const int P = idx / npair, R = idx % npair;
-                if (R < P) continue;
-                const double sp = ss[P], sr = ss[R];
-                if (sp == 0.0 && sr == 0.0) continue;
-                int p, q, x, y;
-                rr_pair(r, P, n2, p, q);
-                rr_pair(r, R, n2, x, y);
-                if (p >= n || q >= n || x >= n || y >= n) continue;
-                const double cp = cc[P], cr = cc[R];
-                if (P == R) {
-                    const double apq = A.at(p, q);
-                    const double t = sp / cp;
-                    A.at(p, p) = A.at(p, p) - t * apq;
-                    A.at(q, q) = A.at(q, q) + t * apq;
-                    A.at(p, q) = 0.0;
-                } else {
-                    const double m00 = A.at(p, x), m01 = A.at(p, y);
-                    const double m10 = A.at(q, x), m11 = A.at(q, y);
-                    // rows: J_P^T M
-                    const double r00 = cp * m00 - sp * m10, r01 = cp * m01 - sp * m11;
-                    const double r10 = sp * m00 + cp * m10, r11 = sp * m01 + cp * m11;
-                    // cols: (.) J_R
-                    A.at(p, x) = r00 * cr - r01 * sr;
-                    A.at(p, y) = r00 * sr + r01 * cr;
-                    A.at(q, x) = r10 * cr - r11 * sr;
-                    A.at(q, y) = r10 * sr + r11 * cr;
+            // 2a. A <- J^T A J on 2x2 blocks (P <= R): a wave per block row P
+            for (int P = w; P < npair; P += NWAVE) {
+                const int pq = pqs[P];
+                const int p = pq & 0xffff, q = pq >> 16;
+                if (p >= n || q >= n) continue;
+                const double sp = ss[P], cp = cc[P];
+                for (int R = P + lane; R < npair; R += WAVE) {
+                    const double sr = ss[R];
+                    if (sp == 0.0 && sr == 0.0) continue;
+                    const int xy = pqs[R];
+                    const int x = xy & 0xffff, y = xy >> 16;
+                    if (x >= n || y >= n) continue;
+                    const double cr = cc[R];
+                    if (P == R) {
+                        const double apq = A.at(p, q);
+                        const double t = sp / cp;
+                        A.at(p, p) = A.at(p, p) - t * apq;
+                        A.at(q, q) = A.at(q, q) + t * apq;
+                        A.at(p, q) = 0.0;
+                    } else {
+                        const double m00 = A.at(p, x), m01 = A.at(p, y);
+                        const double m10 = A.at(q, x), m11 = A.at(q, y);
+                        // rows: J_P^T M
+                        const double r00 = cp * m00 - sp * m10, r01 = cp * m01 - sp * m11;
+                        const double r10 = sp * m00 + cp * m10, r11 = sp * m01 + cp * m11;
+                        // cols: (.) J_R
+                        A.at(p, x) = r00 * cr - r01 * sr;
+                        A.at(p, y) = r00 * sr + r01 * cr;
+                        A.at(q, x) = r10 * cr - r11 * sr;
+                        A.at(q, y) = r10 * sr + r11 * cr;
+                    }
                 }
             }
-            // 2b. vectors: rows p, q of Ct
-            const int nvec = npair * n;
-            for (int idx = threadIdx.x; idx < nvec; idx += NT) {
-                const int k = idx / n, mu = idx % n;
+            // 2b. vectors: rows p, q of Ct, a wave per rotation
+            for (int k = w; k < npair; k += NWAVE) {
                 const double s = ss[k];
                 if (s == 0.0) continue;
-                int p, q;
-                rr_pair(r, k, n2, p, q);
+                const int pq = pqs[k];
+                const int p = pq & 0xffff, q = pq >> 16;
                 if (p >= n || q >= n) continue;
                 const double c = cc[k];
-                double& vp = Ct.at(p, mu);
-                double& vq = Ct.at(q, mu);
-                const double a = vp, b = vq;
-                vp = c * a - s * b;
-                vq = s * a + c * b;
+                for (int mu = lane; mu < n; mu += WAVE) {
+                    auto& vp = Ct.at(p, mu);
+                    auto& vq = Ct.at(q, mu);
+                    const double a = vp, b = vq;
+                    vp = c * a - s * b;
+                    vq = s * a + c * b;
+                }
             }
             __syncthreads();
         }
@@ -432,7 +358,7 @@ __device__ inline void wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-__device__ inline double& pk(double* M, int i, int j) {
+__device__ inline ldsf& pk(ldsf* M, int i, int j) {
     if (i > j) { int t = i; i = j; j = t; }
     return M[i * JS - (i * (i - 1)) / 2 + (j - i)];
 }
@@ -455,7 +381,7 @@ __device__ inline void rr32(int r, int k, int& p, int& q) {
 // LDS: rrtab[r*16+k] = p | q<<8 for pair k of round r, and blktab[e] =
 // P | R<<8 for the e-th 2x2 block (P <= R) of a round.
 
-__device__ inline void build_schedules(int* tab) {
+__device__ inline void build_schedules(ldsi* tab) {
     for (int e = threadIdx.x; e < (JS - 1) * NPAIR32; e += NT) {
         int p, q;
         rr32(e / NPAIR32, e % NPAIR32, p, q);
@@ -472,11 +398,29 @@ __device__ inline void build_schedules(int* tab) {
 // rows/columns are zero are never rotated).  Qt (JS x JS, Qt[col][row])
 // accumulates the rotations so that M_final = Q^T M_0 Q.  Returns 1 if any
 // rotation was applied, 0 if M was already diagonal to `tol` (Q = I).
-__device__ __noinline__ int wave_jacobi_sub(double* M, double* Qt, double* cs, const int* sched,
+//
+// A round is two LDS round trips per lane: the 16 rotation lanes read their
+// pivots and publish (c, s); then every lane gathers the schedule entries,
+// (c, s) and matrix/vector elements of all its work items at once, computes,
+// and stores.  Work items: the 2x2 blocks lane, lane+64, lane+128 (of 136)
+// of the two-sided update, and 8 (pair, row) elements of Qt.  Items touch
+// disjoint elements, so no ordering among them is needed.
+__device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const ldsi* sched,
                                             double tol, int max_sweeps) {
     const int lane = threadIdx.x % WAVE;
-    const int* rrtab = sched;
-    const int* blktab = sched + (JS - 1) * NPAIR32;
+    const ldsi* rrtab = sched;
+    const ldsi* blktab = sched + (JS - 1) * NPAIR32;
+    constexpr int NB = (NBLK32 + WAVE - 1) / WAVE;  // 3 block slots per lane
+    constexpr int NQ = NPAIR32 / (WAVE / JS);       // 8 Qt items per lane
+    int bP[NB], bR[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j) {
+        const int e = lane + WAVE * j;
+        const int PR = (e < NBLK32) ? blktab[e] : -1;
+        bP[j] = (e < NBLK32) ? (PR & 255) : -1;
+        bR[j] = (e < NBLK32) ? (PR >> 8) : -1;
+    }
+    const int qrow = lane % JS, qk0 = lane / JS;
     for (int e = lane; e < JS * JS; e += WAVE) Qt[e] = ((e / JS) == (e % JS)) ? 1.0 : 0.0;
     wave_sync();
     const double skip = 1e-3 * tol;
@@ -491,14 +435,14 @@ __device__ __noinline__ int wave_jacobi_sub(double* M, double* Qt, double* cs, c
         if (off <= tol) break;
         rotated = 1;
         for (int r = 0; r < JS - 1; ++r) {
-            const int* rr = rrtab + r * NPAIR32;
+            const ldsi* rr = rrtab + r * NPAIR32;
             if (lane < NPAIR32) {
                 const int pq = rr[lane];
                 const int p = pq & 255, q = pq >> 8;
                 double c = 1.0, s = 0.0;
-                const double apq = pk(M, p, q);
+                const double apq = pk(M, p, q), app = pk(M, p, p), aqq = pk(M, q, q);
                 if (fabs(apq) > skip) {
-                    const double theta = (pk(M, q, q) - pk(M, p, p)) / (2.0 * apq);
+                    const double theta = (aqq - app) / (2.0 * apq);
                     double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
                     if (theta < 0.0) t = -t;
                     c = 1.0 / sqrt(fma(t, t, 1.0));
@@ -508,42 +452,87 @@ __device__ __noinline__ int wave_jacobi_sub(double* M, double* Qt, double* cs, c
                 cs[JB + lane] = s;
             }
             wave_sync();
-            for (int e = lane; e < NBLK32; e += WAVE) {
-                const int PR = blktab[e];
-                const int P = PR & 255, R = PR >> 8;
-                const double sp = cs[JB + P], sr = cs[JB + R];
-                if (sp == 0.0 && sr == 0.0) continue;
-                const int pq = rr[P], xy = rr[R];
-                const int p = pq & 255, q = pq >> 8, x = xy & 255, y = xy >> 8;
-                const double cp = cs[P], cr = cs[R];
-                if (P == R) {
-                    const double apq = pk(M, p, q);
+            // gather: schedule entries and rotation parameters
+            int bpq[NB], bxy[NB];
+            double bcp[NB], bsp[NB], bcr[NB], bsr[NB];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int P = bP[j] < 0 ? 0 : bP[j], R = bR[j] < 0 ? 0 : bR[j];
+                bpq[j] = rr[P];
+                bxy[j] = rr[R];
+                bcp[j] = cs[P];
+                bsp[j] = cs[JB + P];
+                bcr[j] = cs[R];
+                bsr[j] = cs[JB + R];
+            }
+            int qpq[NQ];
+            double qc[NQ], qs[NQ];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                const int k = qk0 + 2 * j;
+                qpq[j] = rr[k];
+                qc[j] = cs[k];
+                qs[j] = cs[JB + k];
+            }
+            // gather: matrix and vector elements
+            double m[NB][4];
+            int mi[NB][4];
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const int p = bpq[j] & 255, q = bpq[j] >> 8, x = bxy[j] & 255, y = bxy[j] >> 8;
+                const bool diag = bP[j] == bR[j];
+                int i0 = diag ? p : p, j0 = diag ? p : x;
+                int i1 = diag ? q : p, j1 = diag ? q : y;
+                int i2 = diag ? p : q, j2 = diag ? q : x;
+                int i3 = diag ? p : q, j3 = diag ? q : y;
+                mi[j][0] = (i0 <= j0) ? i0 * JS - (i0 * (i0 - 1)) / 2 + (j0 - i0)
+                                      : j0 * JS - (j0 * (j0 - 1)) / 2 + (i0 - j0);
+                mi[j][1] = (i1 <= j1) ? i1 * JS - (i1 * (i1 - 1)) / 2 + (j1 - i1)
+                                      : j1 * JS - (j1 * (j1 - 1)) / 2 + (i1 - j1);
+                mi[j][2] = (i2 <= j2) ? i2 * JS - (i2 * (i2 - 1)) / 2 + (j2 - i2)
+                                      : j2 * JS - (j2 * (j2 - 1)) / 2 + (i2 - j2);
+                mi[j][3] = (i3 <= j3) ? i3 * JS - (i3 * (i3 - 1)) / 2 + (j3 - i3)
+                                      : j3 * JS - (j3 * (j3 - 1)) / 2 + (i3 - j3);
+#pragma unroll
+                for (int u = 0; u < 4; ++u) m[j][u] = M[mi[j][u]];
+            }
+            double qa[NQ], qb[NQ];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                const int p = qpq[j] & 255, q = qpq[j] >> 8;
+                qa[j] = Qt[p * JS + qrow];
+                qb[j] = Qt[q * JS + qrow];
+            }
+            // compute and store.  Diagonal block (P == R): elements
+            // (p,p), (q,q), (p,q); off-diagonal: (p,x), (p,y), (q,x), (q,y).
+#pragma unroll
+            for (int j = 0; j < NB; ++j) {
+                const double sp = bsp[j], sr = bsr[j], cp = bcp[j], cr = bcr[j];
+                if (bP[j] < 0 || (sp == 0.0 && sr == 0.0)) continue;
+                if (bP[j] == bR[j]) {
+                    const double apq = m[j][2];
                     const double t = sp / cp;
-                    pk(M, p, p) -= t * apq;
-                    pk(M, q, q) += t * apq;
-                    pk(M, p, q) = 0.0;
+                    M[mi[j][0]] = m[j][0] - t * apq;
+                    M[mi[j][1]] = m[j][1] + t * apq;
+                    M[mi[j][2]] = 0.0;
                 } else {
-                    const double m00 = pk(M, p, x), m01 = pk(M, p, y);
-                    const double m10 = pk(M, q, x), m11 = pk(M, q, y);
+                    const double m00 = m[j][0], m01 = m[j][1], m10 = m[j][2], m11 = m[j][3];
                     const double r00 = cp * m00 - sp * m10, r01 = cp * m01 - sp * m11;
                     const double r10 = sp * m00 + cp * m10, r11 = sp * m01 + cp * m11;
-                    pk(M, p, x) = r00 * cr - r01 * sr;
-                    pk(M, p, y) = r00 * sr + r01 * cr;
-                    pk(M, q, x) = r10 * cr - r11 * sr;
-                    pk(M, q, y) = r10 * sr + r11 * cr;
+                    M[mi[j][0]] = r00 * cr - r01 * sr;
+                    M[mi[j][1]] = r00 * sr + r01 * cr;
+                    M[mi[j][2]] = r10 * cr - r11 * sr;
+                    M[mi[j][3]] = r10 * sr + r11 * cr;
                 }
             }
-            // columns p, q of Q are rows p, q of Qt: lanes walk a row
-            const int row = lane % JS;
-            for (int k = lane / JS; k < NPAIR32; k += WAVE / JS) {
-                const double s = cs[JB + k];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                const double s = qs[j];
                 if (s == 0.0) continue;
-                const int pq = rr[k];
-                const int p = pq & 255, q = pq >> 8;
-                const double c = cs[k];
-                const double a = Qt[p * JS + row], b = Qt[q * JS + row];
-                Qt[p * JS + row] = c * a - s * b;
-                Qt[q * JS + row] = s * a + c * b;
+                const double c = qc[j];
+                const int p = qpq[j] & 255, q = qpq[j] >> 8;
+                Qt[p * JS + qrow] = c * qa[j] - s * qb[j];
+                Qt[q * JS + qrow] = s * qa[j] + c * qb[j];
             }
             wave_sync();
         }
@@ -556,7 +545,7 @@ __device__ __noinline__ int wave_jacobi_sub(double* M, double* Qt, double* cs, c
 // A is symmetric before the round, so the tile is read as A[P_l, P_k]
 // (row segments, coalesced).  Y = A_kl Q_l stays in registers and feeds the
 // second product as its B operand (element r = k-step r).
-__device__ __noinline__ void tile_update(double* A, int n, const double* Qk, const double* Ql,
+__device__ __noinline__ void tile_update(glbf* A, int n, const glbf* Qk, const glbf* Ql,
                                          int Ik, int Jk, int Il, int Jl, int idk, int idl) {
     const int lane = threadIdx.x % WAVE, li = lane % 16, lg = lane / 16;
     int gk[2], gl_[2];
@@ -625,7 +614,7 @@ __device__ __noinline__ void tile_update(double* A, int n, const double* Qk, con
 
 // Vector update on matrix cores (one wavefront): rows P_k of the row-major
 // matrix X (stride ld), columns c0..c0+63:  X[P_k, c] <- Q_k^T X[P_k, c].
-__device__ __noinline__ void rows_update(double* X, int n, size_t ld, int ncol, const double* Qk,
+__device__ __noinline__ void rows_update(glbf* X, int n, size_t ld, int ncol, const glbf* Qk,
                                          int Ik, int Jk, int c0) {
     const int lane = threadIdx.x % WAVE, li = lane % 16, lg = lane / 16;
     v4d acc[2][4];
@@ -670,33 +659,29 @@ __device__ __noinline__ void rows_update(double* X, int n, size_t ld, int ncol, 
 // Qbuf: global scratch >= (nblocks/2 + 1) * JS * JS doubles.  Returns sweeps
 // or -1 when max_sweeps were spent.
 template <class CAcc>
-__device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Qbuf, int max_sweeps, double tol,
-                                            double* lds, double* red, double* tacc, int inner) {
+__device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf, int max_sweeps, double tol,
+                                            ldsf* lds, ldsf* red, double* tacc, int inner) {
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    double* M = lds + w * JWAVE_LDS;
-    double* Q = M + JPK;
-    double* cs = Q + JS * JS;
+    ldsf* M = lds + w * JWAVE_LDS;
+    ldsf* Q = M + JPK;
+    ldsf* cs = Q + JS * JS;
     const int nb = (n + JB - 1) / JB;
     const int nb2 = nb + (nb & 1);
     const int npair = nb2 / 2;
     const int ntile = npair * (npair - 1) / 2;
     const int nch = (n + WAVE - 1) / WAVE;
-    int* ident = reinterpret_cast<int*>(lds + NWAVE * JWAVE_LDS);  // per pair: Q == I
-    int* sched = ident + 128;                                       // subproblem schedules
+    ldsi* ident = reinterpret_cast<ldsi*>(lds + NWAVE * JWAVE_LDS);  // per pair: Q == I
+    ldsi* sched = ident + 128;                                        // subproblem schedules
     build_schedules(sched);
     // symmetrise from the upper triangle
-    for (int e = threadIdx.x; e < n * n; e += NT) {
-        const int i = e / n, j = e % n;
-        if (j < i) A[(size_t)i * n + j] = A[(size_t)j * n + i];
-    }
+    for (int i = w; i < n; i += NWAVE)
+        for (int j = lane; j < i; j += WAVE) A[(size_t)i * n + j] = A[(size_t)j * n + i];
     __syncthreads();
     int sweep = 0;
     for (;;) {
         double off = 0.0;
-        for (int e = threadIdx.x; e < n * n; e += NT) {
-            const int i = e / n, j = e % n;
-            if (j > i) off = fmax(off, fabs(A[(size_t)i * n + j]));
-        }
+        for (int i = w; i < n; i += NWAVE)
+            for (int j = i + 1 + lane; j < n; j += WAVE) off = fmax(off, fabs(A[(size_t)i * n + j]));
         off = wg_max(off, red);
         if (off <= tol) return sweep;
         if (sweep >= max_sweeps) return -1;
@@ -719,7 +704,7 @@ __device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Q
                 }
                 wave_sync();
                 const int rot = wave_jacobi_sub(M, Q, cs, sched, tol, inner);
-                double* qg = Qbuf + (size_t)k * JS * JS;
+                glbf* qg = Qbuf + (size_t)k * JS * JS;
                 if (lane == 0) ident[k] = rot ? 0 : 1;
                 if (rot) {
                     for (int e = lane; e < JS * JS; e += WAVE) {
@@ -767,20 +752,20 @@ __device__ __noinline__ int wg_block_jacobi(double* A, int n, CAcc Ct, double* Q
 }
 
 struct RowsGlobal {
-    double* a;
+    glbf* a;
     int ld;
-    __device__ inline double& at(int k, int mu) const { return a[(size_t)k * ld + mu]; }
+    __device__ inline glbf& at(int k, int mu) const { return a[(size_t)k * ld + mu]; }
 };
 
 struct RowsLds {
-    double* a;
+    ldsf* a;
     int ld;
-    __device__ inline double& at(int k, int mu) const { return a[k * ld + mu]; }
+    __device__ inline ldsf& at(int k, int mu) const { return a[k * ld + mu]; }
 };
 
 // Ascending bitonic sort of (key, idx) pairs in LDS; len is a power of two
 // and entries beyond the valid range must hold +inf keys.
-__device__ inline void wg_bitonic_sort(double* key, int* idx, int len) {
+__device__ inline void wg_bitonic_sort(ldsf* key, ldsi* idx, int len) {
     for (int size = 2; size <= len; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {
             for (int t = threadIdx.x; t < len / 2; t += NT) {
