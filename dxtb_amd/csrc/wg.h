@@ -132,16 +132,17 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) acc[i][j] = v4zero();
-        for (int k0 = 0; k0 < K; k0 += 16) {
+        // operand panels k0..k0+15 go global -> registers -> LDS; the next
+        // panel's global loads are issued before the current panel's MFMAs
+        double pa[E], pb[E];
+        auto fetch = [&](int k0) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 int m, k;
                 if (TA) { k = t / (TILE / E); m = (t % (TILE / E)) * E + e; }
                 else { m = t / (16 / E); k = (t % (16 / E)) * E + e; }
                 const int gm = m0 + m, gk = k0 + k;
-                double v = 0.0;
-                if (gm < M && gk < K) v = TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk];
-                As[k * LDT + m] = v;
+                pa[e] = (gm < M && gk < K) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
@@ -149,11 +150,27 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                 if (TB) { n = t / (16 / E); k = (t % (16 / E)) * E + e; }
                 else { k = t / (TILE / E); n = (t % (TILE / E)) * E + e; }
                 const int gn = n0 + n, gk = k0 + k;
-                double v = 0.0;
-                if (gn < N && gk < K) v = TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn];
-                Bs[k * LDT + n] = v;
+                pb[e] = (gn < N && gk < K) ? (TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn]) : 0.0;
+            }
+        };
+        fetch(0);
+        for (int k0 = 0; k0 < K; k0 += 16) {
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int m, k;
+                if (TA) { k = t / (TILE / E); m = (t % (TILE / E)) * E + e; }
+                else { m = t / (16 / E); k = (t % (16 / E)) * E + e; }
+                As[k * LDT + m] = pa[e];
+            }
+#pragma unroll
+            for (int e = 0; e < E; ++e) {
+                int n, k;
+                if (TB) { n = t / (16 / E); k = (t % (16 / E)) * E + e; }
+                else { k = t / (TILE / E); n = (t % (TILE / E)) * E + e; }
+                Bs[k * LDT + n] = pb[e];
             }
             __syncthreads();
+            if (k0 + 16 < K) fetch(k0 + 16);
 #pragma unroll
             for (int kk = 0; kk < 4; ++kk) {
                 double a[FM], b[FN];
