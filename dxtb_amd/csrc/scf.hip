@@ -242,6 +242,34 @@ __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, doubl
 }
 
 // ---------------------------------------------------------------------------
+// Occupied orbitals for the density products.  Sorted by energy the orbitals
+// with f > 0 form a prefix of length kocc; G[j] = C_{sidx[j]} (row j) and
+// H[j] = w_j G[j] with w = f (or f e when `with_e`).  H may alias Ct: it is
+// written only after every row of G has been gathered.  Returns kocc.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ int gather_occupied(const Mol& M, const Shared& sh, glbf* G, glbf* H,
+                                            bool with_e) {
+    const int n = M.n;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    int k = 0;
+    for (int j = threadIdx.x; j < n; j += NT)
+        if (sh.ftot[sh.sidx[j]] > 0.0) k = j + 1;
+    const int kocc = (int)wg_max((double)k, sh.red);
+    for (int j = w; j < kocc; j += NWAVE) {
+        const glbf* src = M.Ct + (size_t)sh.sidx[j] * n;
+        for (int c = lane; c < n; c += WAVE) G[(size_t)j * n + c] = src[c];
+    }
+    __syncthreads();
+    for (int j = w; j < kocc; j += NWAVE) {
+        const int o = sh.sidx[j];
+        const double f = with_e ? sh.ftot[o] * sh.ev[o] : sh.ftot[o];
+        for (int c = lane; c < n; c += WAVE) H[(size_t)j * n + c] = f * G[(size_t)j * n + c];
+    }
+    __syncthreads();
+    return kocc;
+}
+
+// ---------------------------------------------------------------------------
 // one SCF map v_in -> v_out (scf/base.py:651-675 iterate_potential);
 // leaves P, q, e_orb, sorted e/occupations and shell/atom charges behind.
 // ---------------------------------------------------------------------------
@@ -284,14 +312,10 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     ok = fermi_channel(M, sh, M.nel_b, o.kt, o.fermi_thresh, o.fermi_maxiter, o.padded != 0,
                        M.occ + M.nvec_total, &ef[1]) && ok;
     if (!ok) *status |= XTB_MOL_FERMI;
-    // P = C diag(f) C^T
-    for (int r = w; r < n; r += NWAVE) {
-        const double f = sh.ftot[r];
-        for (int c = lane; c < n; c += WAVE) M.T[(size_t)r * n + c] = f * M.Ct[(size_t)r * n + c];
-    }
-    __syncthreads();
+    // P = C diag(f) C^T over the occupied orbitals (F and T are free here)
     t0 = now_us();
-    gemm<true, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.P, n, sh.big);
+    const int kocc = gather_occupied(M, sh, M.F, M.T, false);
+    gemm<true, false>(n, n, kocc, 1.0, M.F, n, M.T, n, 0.0, M.P, n, sh.big);
     sh.tacc[1] += now_us() - t0;
     // Mulliken charges and orbital energies (scf/base.py:764-815), one wave per row
     {
@@ -599,14 +623,10 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     wg_sum_n<4>(acc, sh.red);
     for (int j = threadIdx.x; j < M.n; j += NT) M.emo[j] = sh.skey[j];
     if (o.want_w) {
-        // W = C diag(f e) C^T (analytical.py:186-190)
-        for (int k = threadIdx.x / WAVE; k < M.n; k += NWAVE) {
-            const double fe = sh.ftot[k] * sh.ev[k];
-            for (int c = threadIdx.x % WAVE; c < M.n; c += WAVE)
-                M.F[(size_t)k * M.n + c] = fe * M.Ct[(size_t)k * M.n + c];
-        }
-        __syncthreads();
-        gemm<true, false>(M.n, M.n, M.n, 1.0, M.Ct, M.n, M.F, M.n, 0.0, M.T, M.n, sh.big);
+        // W = C diag(f e) C^T (analytical.py:186-190) over the occupied
+        // orbitals; the orbitals are not needed afterwards, so Ct takes f e C
+        const int kocc = gather_occupied(M, sh, M.F, M.Ct, true);
+        gemm<true, false>(M.n, M.n, kocc, 1.0, M.F, M.n, M.Ct, M.n, 0.0, M.T, M.n, sh.big);
     }
     if (threadIdx.x == 0) {
         const double escf = acc[0] + acc[1] + acc[2];

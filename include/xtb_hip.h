@@ -183,7 +183,8 @@ typedef struct xtb_scf_opts {
 typedef struct xtb_scf_state {
     double* F;      /* packed n x n: Fock / transformed Fock */
     double* C;      /* packed n x n: MO coefficients stored by vector,
-                       C[k*n + mu] = coefficient of orbital mu in MO k */
+                       C[k*n + mu] = coefficient of orbital mu in MO k;
+                       scratch on exit when want_w */
     double* T;      /* packed n x n: scratch; energy-weighted density W on exit if want_w */
     double* P;      /* packed n x n: density */
     double* pot;    /* [nvec] orbital potential of the final charges */
