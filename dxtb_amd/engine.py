@@ -189,7 +189,7 @@ class ScfOptions:
     diag_offset: float = 0.01
     jacobi_max_sweeps: int = 40
     jacobi_tol: float = 1e-13
-    jacobi_inner: int = 2
+    jacobi_inner: int = 1
     eig_inexact: float = 1e-2
 
     def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:

@@ -84,27 +84,44 @@ def tc2(Fp: torch.Tensor, nocc: int, maxiter: int = 120) -> torch.Tensor:
 
 
 def lanczos_extreme(A: torch.Tensor, proj: torch.Tensor, largest: bool, steps: int = 60) -> float:
-    """Extreme eigenvalue of proj A proj on range(proj) by Lanczos with full
-    reorthogonalisation (Ritz values bound the spectrum from inside)."""
+    """Extreme eigenvalue of A on range(proj) (proj an orthogonal projector
+    commuting with A): Lanczos with full reorthogonalisation on
+    B = proj A proj + c (I - proj), where c puts the complement below (largest)
+    or above (smallest) the whole spectrum of A (Gershgorin bounds), so the
+    wanted value is the extreme eigenvalue of B on the full space.  Ritz values
+    bound it from inside."""
     n = A.shape[0]
+    d = torch.diagonal(A)
+    r = A.abs().sum(1) - d.abs()
+    lo, hi = float((d - r).min()), float((d + r).max())
+    c = lo - 1.0 if largest else hi + 1.0
+    tiny = 1e-10 * (hi - lo + 1.0)
+
+    def op(x):
+        px = proj @ x
+        return proj @ (A @ px) + c * (x - px)
+
     g = torch.Generator(device="cpu").manual_seed(1234)
     v = torch.randn(n, generator=g, dtype=A.dtype).to(A.device)
-    v = proj @ v
     v = v / v.norm()
     V = [v]
     alphas, betas = [], []
-    for k in range(steps):
-        w = proj @ (A @ V[-1])
+    for k in range(min(steps, n)):
+        w = op(V[-1])
         a = float(w @ V[-1])
         alphas.append(a)
-        for u in V:
-            w = w - (w @ u) * u
+        # two Gram-Schmidt passes ("twice is enough") keep V orthonormal when
+        # the Krylov space runs out (B has few distinct eigenvalues)
+        for _ in range(2):
+            for u in V:
+                w = w - (w @ u) * u
         b = float(w.norm())
-        if b < 1e-12 or k == steps - 1:
+        if b < tiny or k == min(steps, n) - 1:
             break
         betas.append(b)
         V.append(w / b)
-    T = np.diag(alphas) + np.diag(betas[: len(alphas) - 1], 1) + np.diag(betas[: len(alphas) - 1], -1)
+    m = len(alphas)
+    T = np.diag(alphas) + np.diag(betas[: m - 1], 1) + np.diag(betas[: m - 1], -1)
     ev = np.linalg.eigvalsh(T)
     return float(ev[-1] if largest else ev[0])
 

@@ -3,6 +3,7 @@
 import ctypes as ct
 
 import numpy as np
+import torch
 import pytest
 
 import dxtb_amd._lib as L
@@ -88,3 +89,33 @@ def test_sto_expansion_identical_to_oracle():
         a, c = slater_to_gauss(ng, n, l, 1.7)
         ao, co = og.slater_to_gauss(ng, n, l, 1.7)
         assert np.allclose(a, ao.numpy(), rtol=1e-15, atol=0) and np.allclose(c, co.numpy(), rtol=1e-15, atol=0)
+
+
+def _sym_with_spectrum(ev, seed):
+    g = torch.Generator().manual_seed(seed)
+    n = len(ev)
+    Q, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64))
+    return Q @ torch.diag(torch.as_tensor(ev, dtype=torch.float64)) @ Q.T
+
+
+def test_purification_projector_and_gap_estimates():
+    """Whole-GPU path helpers (dxtb_amd/large.py) on the CPU: TC2 gives the
+    spectral projector onto the nocc lowest eigenvectors, and the Lanczos
+    estimates of the frontier eigenvalues bound them from inside.  The second
+    case has few occupied states (the Krylov space of the projected operator
+    runs out long before the step limit)."""
+    from dxtb_amd.large import lanczos_extreme, tc2
+
+    for n, nocc, seed in ((200, 70, 0), (94, 8, 1)):
+        ev = np.sort(np.random.default_rng(seed).normal(size=n))
+        ev[nocc:] += 0.4
+        F = _sym_with_spectrum(ev, seed)
+        D = tc2(F, nocc)
+        w, V = torch.linalg.eigh(F)
+        P = V[:, :nocc] @ V[:, :nocc].T
+        assert float((D - P).abs().max()) < 1e-12
+        eye = torch.eye(n, dtype=torch.float64)
+        homo = lanczos_extreme(F, D, largest=True)
+        lumo = lanczos_extreme(F, eye - D, largest=False)
+        assert homo <= float(w[nocc - 1]) + 1e-12 and homo > float(w[nocc - 1]) - 1e-4
+        assert lumo >= float(w[nocc]) - 1e-12 and lumo < float(w[nocc]) + 1e-4
