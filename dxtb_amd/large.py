@@ -12,7 +12,9 @@ a dense FP64 GEMM on the matrix cores (rocBLAS through torch.matmul):
   occupations at the electronic temperature are exactly 0 or 1 in FP64, so
   this is the same density upstream builds from its eigenvectors
   (src/dxtb/_src/scf/base.py:861-907).  A Lanczos estimate of the
-  HOMO-LUMO gap verifies that; below 80 kT the system is refused;
+  HOMO-LUMO gap verifies that: with the Fermi level mid-gap, a gap of
+  60 kT leaves occupation tails below 1/(1 + e^30) ~ 1e-13; below that the
+  system is refused;
 * energy-weighted density W = P F P / 2 per closed shell, for the analytic
   gradient kernel.
 
@@ -32,6 +34,10 @@ from . import _lib
 from .constants import KELVIN2AU
 from .engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
 from .param import SpeciesTables
+
+# smallest HOMO-LUMO gap (in kT) for which 0/1 occupations equal the
+# Fermi-Dirac ones to ~1e-13
+GAP_MIN_KT = 60.0
 
 
 def newton_schulz_inv_sqrt(S: torch.Tensor, tol: float = 1e-14, maxiter: int = 60) -> torch.Tensor:
@@ -234,7 +240,7 @@ class LargeSystem:
         homo = lanczos_extreme(Fp, Da, largest=True)
         lumo = lanczos_extreme(Fp, eye - Da, largest=False)
         gap = lumo - homo
-        if gap < 80.0 * kt:
+        if gap < GAP_MIN_KT * kt:
             raise RuntimeError(f"HOMO-LUMO gap {gap:.3e} Eh too small for the purification path")
         e_h0 = float(eorb.sum())
         e_es2 = float(0.5 * (qsh * gq).sum())

@@ -122,7 +122,11 @@ class RaggedServer:
             waited = now - q[0].arrival >= self.max_wait
             if not (full or waited or not self.inflight):
                 continue
-            if best is None or q[0].arrival < best_t:
+            # oldest request first; among requests that arrived together
+            # (within max_wait) the larger class first, so that the longest
+            # launches start earliest
+            if best is None or q[0].arrival < best_t - self.max_wait or (
+                    q[0].arrival <= best_t + self.max_wait and k > best):
                 best, best_t = k, q[0].arrival
         return best
 
@@ -190,9 +194,14 @@ class RaggedServer:
                 keep.append(f)
         self.inflight = keep
         if block and not out and self.inflight:
-            oldest = min(self.inflight, key=lambda f: f.t_launch)
-            self.inflight.remove(oldest)
-            out += self._retire(oldest)
+            # wait for whichever launch finishes first
+            while not out:
+                time.sleep(2e-4)
+                for f in self.inflight:
+                    if f.event.query():
+                        self.inflight.remove(f)
+                        out += self._retire(f)
+                        break
         return out
 
     def step(self, now: float | None = None) -> list[Request]:
