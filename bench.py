@@ -150,6 +150,8 @@ def main() -> int:
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--profile-run", action="store_true",
                     help="serve one chunk only (for rocprofv3 runs), print nothing else")
+    ap.add_argument("--profile-serving", action="store_true",
+                    help="serve one chunk through the 4-stream loop only (kernel timelines)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -188,6 +190,11 @@ def main() -> int:
         print(json.dumps({"profile_run": rp}))
         return 0
     server = RaggedServer(dev, opts=opts)
+    if a.profile_serving:
+        server.run_stream(_requests(stream, 0, a.chunk), realtime=False)  # warm-up
+        r = server.run_stream(_requests(stream, a.chunk, a.chunk), realtime=False)
+        print(json.dumps({"profile_serving": r}))
+        return 0
     pos = 0
     for w in range(a.warmup):
         server.run_stream(_requests(stream, pos, a.chunk), realtime=False)

@@ -78,7 +78,7 @@ __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 // to mol.Ct.  Returns Jacobi sweeps (-1 on failure).
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_sweeps, double rel_tol,
-                                     double loose, int inner) {
+                                     double loose, int inner, int cb_lo, int cb_hi) {
     const int n = M.n;
     // scale for the absolute tolerance
     double dmax = 0.0;
@@ -97,8 +97,8 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
             for (int idx = threadIdx.x; idx < n * n; idx += NT) cbuf[idx] = M.Ct[idx];
         }
         __syncthreads();
-        if (cl) sweeps = wg_jacobi(A, n, RowsLds{cbuf, n}, max_sweeps, tol, sh.cs, sh.red);
-        else sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red);
+        if (cl) sweeps = wg_jacobi(A, n, RowsLds{cbuf, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi);
+        else sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = A.at(i, i);
         if (cl) {
             for (int idx = threadIdx.x; idx < n * n; idx += NT) M.Ct[idx] = cbuf[idx];
@@ -106,11 +106,11 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
     } else if (sh.big_cap >= NBLOCK_LDS) {
         // block Jacobi; T is free scratch here and holds the subproblem rotations
         sweeps = wg_block_jacobi(M.F, n, RowsGlobal{M.Ct, n}, M.T, max_sweeps, tol, sh.big, sh.red,
-                                 sh.tacc, inner);
+                                 sh.tacc, inner, cb_lo, cb_hi);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = M.F[(size_t)i * n + i];
     } else {
         SymFull A{M.F, n};
-        sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red);
+        sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = A.at(i, i);
     }
     __syncthreads();
@@ -242,20 +242,21 @@ __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, doubl
 }
 
 // ---------------------------------------------------------------------------
-// Occupied orbitals for the density products.  Sorted by energy the orbitals
-// with f > 0 form a prefix of length kocc; G[j] = C_{sidx[j]} (row j) and
-// H[j] = w_j G[j] with w = f (or f e when `with_e`).  H may alias Ct: it is
-// written only after every row of G has been gathered.  Returns kocc.
+// Orbitals in ascending energy order for the density products.  The
+// orbitals with f > 0 form a prefix of length kocc; G[j] = C_{sidx[j]} (row
+// j, for j < kocc, or every j when `all_rows`) and H[j] = w_j G[j] (j < kocc)
+// with w = f (or f e when `with_e`).  H may alias Ct: it is written only
+// after every row of G has been gathered.  Returns kocc.
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int gather_occupied(const Mol& M, const Shared& sh, glbf* G, glbf* H,
-                                            bool with_e) {
+                                            bool with_e, bool all_rows = false) {
     const int n = M.n;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     int k = 0;
     for (int j = threadIdx.x; j < n; j += NT)
         if (sh.ftot[sh.sidx[j]] > 0.0) k = j + 1;
     const int kocc = (int)wg_max((double)k, sh.red);
-    for (int j = w; j < kocc; j += NWAVE) {
+    for (int j = w; j < (all_rows ? n : kocc); j += NWAVE) {
         const glbf* src = M.Ct + (size_t)sh.sidx[j] * n;
         for (int c = lane; c < n; c += WAVE) G[(size_t)j * n + c] = src[c];
     }
@@ -275,8 +276,25 @@ __device__ __noinline__ int gather_occupied(const Mol& M, const Shared& sh, glbf
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
                         const xtb_params& par, const xtb_scf_opts& o, const glbf* vin,
-                        glbf* vout, int* status, double* ef, double loose) {
+                        glbf* vout, int* status, double* ef, double loose, bool allow_bd,
+                        bool sort_rows) {
     const int n = M.n;
+    // Occupation classes for this step's eigensolve.  With integer alpha/beta
+    // counts and frontier gaps of at least bd_gap_kt x kT in the previous
+    // step's spectrum (sh.skey, ascending; the rows of Ct are in that order),
+    // only the coupling between doubly, singly and un-occupied orbitals is
+    // removed: that fixes the density of 0/1 occupations, whose Fermi tails
+    // at such gaps are below 1/(1 + e^(bd_gap_kt/2)).
+    int cb_lo = -1, cb_hi = -1;
+    if (allow_bd && o.bd_gap_kt > 0.0 && !o.padded) {
+        const double na = M.nel_a, nb = M.nel_b;
+        const int ia = (int)rint(na), ib = (int)rint(nb);
+        bool ok = fabs(na - ia) < 1e-12 && fabs(nb - ib) < 1e-12 && ib <= ia;
+        const double need = o.bd_gap_kt * o.kt;
+        if (ok && ia > 0 && ia < n) ok = sh.skey[ia] - sh.skey[ia - 1] >= need;
+        if (ok && ib > 0 && ib < n) ok = sh.skey[ib] - sh.skey[ib - 1] >= need;
+        if (ok) { cb_lo = ib; cb_hi = ia; }
+    }
     const size_t nn = (size_t)n * n;
     // Fock build (scf/base.py:817-859)
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -293,7 +311,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
     gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
     double t1 = now_us();
-    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner);
+    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi);
     double t2 = now_us();
     sh.tacc[1] += t1 - t0;
     sh.tacc[2] += t2 - t1;
@@ -314,8 +332,13 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     if (!ok) *status |= XTB_MOL_FERMI;
     // P = C diag(f) C^T over the occupied orbitals (F and T are free here)
     t0 = now_us();
-    const int kocc = gather_occupied(M, sh, M.F, M.T, false);
+    const int kocc = gather_occupied(M, sh, M.F, M.T, false, sort_rows);
     gemm<true, false>(n, n, kocc, 1.0, M.F, n, M.T, n, 0.0, M.P, n, sh.big);
+    if (sort_rows) {
+        // the next step starts from the orbitals in ascending energy order
+        for (int r = w; r < n; r += NWAVE)
+            for (int c = lane; c < n; c += WAVE) M.Ct[(size_t)r * n + c] = M.F[(size_t)r * n + c];
+    }
     sh.tacc[1] += now_us() - t0;
     // Mulliken charges and orbital energies (scf/base.py:764-815), one wave per row
     {
@@ -577,7 +600,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     // step, from which every reported quantity comes, to full tolerance.
     double resid = 1e-2;
     const int sw_first = scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef,
-                                  o.eig_inexact > 0.0 ? 1e-4 : 0.0);
+                                  o.eig_inexact > 0.0 ? 1e-4 : 0.0, false, true);
     sweeps += sw_first;
     ++iters;
     if (o.mix_guess) {
@@ -588,7 +611,8 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     }
     bool converged = false;
     for (int it = 0; it < o.maxiter; ++it) {
-        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, o.eig_inexact * resid);
+        sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, o.eig_inexact * resid,
+                           o.eig_inexact > 0.0, true);
         ++iters;
         const bool c = anderson(mx, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, &resid);
         if (c) { converged = true; break; }
@@ -597,7 +621,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     if (o.maxiter == 0) converged = true;
     // converged_to_charges (scf/base.py:472-513): density of the last SCF
     // output potential; the potential of its charges goes to st.pot
-    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0);
+    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false, false);
     sweeps += sw_last;
 
     // energies (scf/base.py:514-623)

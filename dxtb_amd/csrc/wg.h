@@ -252,9 +252,17 @@ __device__ inline void rr_pair(int r, int k, int n2, int& p, int& q) {
 // each rotation of the round.  Loops are laid out wave x lane so that no
 // index needs an integer division.  Returns the number of sweeps, or -1
 // when max_sweeps were spent without reaching `tol` (absolute).
+//
+// Occupation classes (cb_lo >= 0): index i has class (i < cb_lo) + (i < cb_hi)
+// (doubly, singly, un-occupied for sorted orbitals).  Only pairs of different
+// classes are rotated and tested, which decouples the classes without
+// diagonalising inside them; the density of integer occupations needs no more.
+__device__ inline int occ_class(int i, int cb_lo, int cb_hi) { return (i < cb_lo) + (i < cb_hi); }
+
 template <class Sym, class CAcc>
 __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf* cs,
-                         ldsf* red) {
+                         ldsf* red, int cb_lo = -1, int cb_hi = -1) {
+    const bool bd = cb_lo >= 0;
     const int n2 = n + (n & 1);
     const int npair = n2 / 2;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -265,8 +273,11 @@ __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf
     for (;;) {
         // convergence test: largest off-diagonal element
         double off = 0.0;
-        for (int i = w; i < n; i += NWAVE)
-            for (int j = i + 1 + lane; j < n; j += WAVE) off = fmax(off, fabs(A.at(i, j)));
+        for (int i = w; i < n; i += NWAVE) {
+            const int ci = occ_class(i, cb_lo, cb_hi);
+            for (int j = i + 1 + lane; j < n; j += WAVE)
+                if (!bd || occ_class(j, cb_lo, cb_hi) != ci) off = fmax(off, fabs(A.at(i, j)));
+        }
         off = wg_max(off, red);
         if (off <= tol) return sweep;
         if (sweep >= max_sweeps) return -1;
@@ -277,7 +288,7 @@ __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf
                 int p, q;
                 rr_pair(r, k, n2, p, q);
                 double c = 1.0, s = 0.0;
-                if (p < n && q < n) {
+                if (p < n && q < n && (!bd || occ_class(p, cb_lo, cb_hi) != occ_class(q, cb_lo, cb_hi))) {
                     const double apq = A.at(p, q);
                     if (fabs(apq) > skip) {
                         const double app = A.at(p, p), aqq = A.at(q, q);
@@ -423,8 +434,11 @@ __device__ inline void build_schedules(ldsi* tab) {
 // of the two-sided update, and 8 (pair, row) elements of Qt.  Items touch
 // disjoint elements, so no ordering among them is needed.
 __device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const ldsi* sched,
-                                            double tol, int max_sweeps) {
+                                            double tol, int max_sweeps, bool bd, unsigned lo_mask,
+                                            unsigned hi_mask) {
     const int lane = threadIdx.x % WAVE;
+    // occupation class of a slot (when bd, only inter-class pairs rotate)
+    auto scls = [&](int i) { return (int)((lo_mask >> i) & 1u) + (int)((hi_mask >> i) & 1u); };
     const ldsi* rrtab = sched;
     const ldsi* blktab = sched + (JS - 1) * NPAIR32;
     constexpr int NB = (NBLK32 + WAVE - 1) / WAVE;  // 3 block slots per lane
@@ -446,7 +460,7 @@ __device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const l
         double off = 0.0;
         for (int e = lane; e < JS * JS; e += WAVE) {
             const int i = e / JS, j = e % JS;
-            if (j > i) off = fmax(off, fabs(pk(M, i, j)));
+            if (j > i && (!bd || scls(i) != scls(j))) off = fmax(off, fabs(pk(M, i, j)));
         }
         off = wave_max(off);
         if (off <= tol) break;
@@ -458,7 +472,7 @@ __device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const l
                 const int p = pq & 255, q = pq >> 8;
                 double c = 1.0, s = 0.0;
                 const double apq = pk(M, p, q), app = pk(M, p, p), aqq = pk(M, q, q);
-                if (fabs(apq) > skip) {
+                if (fabs(apq) > skip && (!bd || scls(p) != scls(q))) {
                     const double theta = (aqq - app) / (2.0 * apq);
                     double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
                     if (theta < 0.0) t = -t;
@@ -677,7 +691,9 @@ __device__ __noinline__ void rows_update(glbf* X, int n, size_t ld, int ncol, co
 // or -1 when max_sweeps were spent.
 template <class CAcc>
 __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf, int max_sweeps, double tol,
-                                            ldsf* lds, ldsf* red, double* tacc, int inner) {
+                                            ldsf* lds, ldsf* red, double* tacc, int inner,
+                                            int cb_lo, int cb_hi) {
+    const bool bd = cb_lo >= 0;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     ldsf* M = lds + w * JWAVE_LDS;
     ldsf* Q = M + JPK;
@@ -697,8 +713,11 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
     int sweep = 0;
     for (;;) {
         double off = 0.0;
-        for (int i = w; i < n; i += NWAVE)
-            for (int j = i + 1 + lane; j < n; j += WAVE) off = fmax(off, fabs(A[(size_t)i * n + j]));
+        for (int i = w; i < n; i += NWAVE) {
+            const int ci = occ_class(i, cb_lo, cb_hi);
+            for (int j = i + 1 + lane; j < n; j += WAVE)
+                if (!bd || occ_class(j, cb_lo, cb_hi) != ci) off = fmax(off, fabs(A[(size_t)i * n + j]));
+        }
         off = wg_max(off, red);
         if (off <= tol) return sweep;
         if (sweep >= max_sweeps) return -1;
@@ -709,6 +728,23 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
             for (int k = w; k < npair; k += NWAVE) {
                 int I, J;
                 rr_pair(r, k, nb2, I, J);
+                unsigned lo_mask = 0u, hi_mask = 0u;
+                if (bd) {
+                    // slot classes; a pair whose valid slots share one class
+                    // has nothing to decouple
+                    const int g = slot_index(I, J, lane % JS, n);
+                    const unsigned long long v = __ballot(lane < JS && g >= 0);
+                    const unsigned long long lo = __ballot(lane < JS && g >= 0 && g < cb_lo);
+                    const unsigned long long hi = __ballot(lane < JS && g >= 0 && g < cb_hi);
+                    const unsigned vm = (unsigned)v;
+                    lo_mask = (unsigned)lo;
+                    hi_mask = (unsigned)hi;
+                    const bool pure = (lo_mask == 0u || lo_mask == vm) && (hi_mask == 0u || hi_mask == vm);
+                    if (pure) {
+                        if (lane == 0) ident[k] = 1;
+                        continue;
+                    }
+                }
                 for (int e = lane; e < JS * JS; e += WAVE) {
                     const int i = e / JS, j = e % JS;
                     if (j >= i) {
@@ -720,7 +756,7 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
                     }
                 }
                 wave_sync();
-                const int rot = wave_jacobi_sub(M, Q, cs, sched, tol, inner);
+                const int rot = wave_jacobi_sub(M, Q, cs, sched, tol, inner, bd, lo_mask, hi_mask);
                 glbf* qg = Qbuf + (size_t)k * JS * JS;
                 if (lane == 0) ident[k] = rot ? 0 : 1;
                 if (rot) {

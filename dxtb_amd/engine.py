@@ -191,6 +191,9 @@ class ScfOptions:
     jacobi_tol: float = 1e-13
     jacobi_inner: int = 1
     eig_inexact: float = 1e-2
+    # intermediate SCF steps decouple occupied from virtual orbitals only
+    # (all that the density needs) when the frontier gaps exceed this many kT
+    bd_gap_kt: float = 60.0
 
     def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
@@ -201,7 +204,7 @@ class ScfOptions:
             jacobi_inner=self.jacobi_inner, eig_inexact=self.eig_inexact,
             x_atol=self.x_atol, x_atol_max=self.x_atol_max, kt=self.fermi_etemp * C.KELVIN2AU,
             damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
-            fermi_thresh=thr, jacobi_tol=self.jacobi_tol,
+            fermi_thresh=thr, jacobi_tol=self.jacobi_tol, bd_gap_kt=self.bd_gap_kt,
         )
 
 

@@ -122,11 +122,10 @@ class RaggedServer:
             waited = now - q[0].arrival >= self.max_wait
             if not (full or waited or not self.inflight):
                 continue
-            # oldest request first; among requests that arrived together
-            # (within max_wait) the larger class first, so that the longest
-            # launches start earliest
-            if best is None or q[0].arrival < best_t - self.max_wait or (
-                    q[0].arrival <= best_t + self.max_wait and k > best):
+            # oldest request first; among requests that arrived at the same
+            # instant (a submitted batch) the larger class first, so that the
+            # longest launches start earliest
+            if best is None or q[0].arrival <= best_t:
                 best, best_t = k, q[0].arrival
         return best
 

@@ -177,6 +177,9 @@ typedef struct xtb_scf_opts {
     double fermi_thresh;
     double jacobi_tol;
     double eig_inexact;      /* eigensolve tolerance = this x last |dv|_inf (0: exact) */
+    double bd_gap_kt;        /* intermediate SCF steps only decouple occupied from
+                                virtual orbitals when the previous step's frontier
+                                gaps are >= this many kT (0: always full eigensolve) */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */
