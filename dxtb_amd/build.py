@@ -19,6 +19,9 @@ OUT = PKG / "libxtbhip.so"
 FLAGS = [
     "-O3", "-fPIC", "-shared", "-std=c++17", "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # mistakes that compile but would misbehave on the device
+    "-Werror=infinite-recursion", "-Werror=return-type", "-Werror=uninitialized",
+    "-Werror=sometimes-uninitialized",
 ]
 
 
@@ -39,6 +42,12 @@ def build(verbose: bool = False) -> Path:
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed")
+    # show compiler warnings (the host pass always notes the device-only
+    # target feature; that one is expected)
+    warn = [ln for ln in (res.stdout + res.stderr).splitlines()
+            if "warning" in ln and "packed-fp32-ops" not in ln and "hip-link" not in ln]
+    if warn:
+        sys.stderr.write("\n".join(warn) + "\n")
     tmp.replace(OUT)
     return OUT
 

@@ -271,6 +271,160 @@ __device__ __noinline__ int gather_occupied(const Mol& M, const Shared& sh, glbf
 }
 
 // ---------------------------------------------------------------------------
+// Fock build F = H0 - 1/2 S o (v_i + v_j) (scf/base.py:817-859)
+// ---------------------------------------------------------------------------
+__device__ inline void fock_build(const Mol& M, const glbf* vin) {
+    const int n = M.n;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    for (int r = w; r < n; r += NWAVE) {
+        const double vr = vin[r];
+        for (int c = lane; c < n; c += WAVE) {
+            const size_t e = (size_t)r * n + c;
+            M.F[e] = M.H0[e] - 0.5 * M.S[e] * (vr + vin[c]);
+        }
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Perturbative occupied-virtual decoupling for intermediate SCF steps of
+// closed shells with a clear gap (pseudo-diagonalisation in the sense of
+// Stewart, Csaszar and Pulay, J. Comput. Chem. 3, 227 (1982), done with
+// matrix-core GEMMs).  The rows of Ct are in ascending energy order, the
+// first k doubly occupied.  With A = C^T F C (only its occupied rows and the
+// diagonal are formed) and e = diag(A), the first-order amplitudes
+// K_ia = A_ia / (e_a - e_i) define the antisymmetric generator
+// G = [[0, K], [-K^T, 0]]; the orbitals are rotated by exp(G), which is
+// exactly orthogonal:
+//   C_occ <- cos(sqrt X) C_occ - sinc(sqrt X) Z
+//   C_vir <- C_vir + K^T [sinc(sqrt X) C_occ + phi(X) Z]
+// with X = K K^T (k x k), Z = K C_vir (k x n), phi(x) = (cos sqrt x - 1)/x;
+// the functions of X are power series evaluated by Horner's rule, truncated
+// once the next term is below 1e-17 (||X||_F bounds the spectral norm).  The
+// coupling left is second order, about max|K| max|A_ov|; a second pass runs
+// when that estimate exceeds the tolerance.
+// Returns 1 when done (rotated, or already decoupled to the tolerance), 0 to
+// ask for the Jacobi eigensolve with Ct and F untouched, -1 to ask for it
+// after F was used as scratch (the caller rebuilds F).  Jacobi is asked for
+// when an amplitude exceeds pd_kmax, a denominator is not positive, k > n/2,
+// or a series would need more than 9 terms.
+// Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the
+// Horner accumulators.
+// ---------------------------------------------------------------------------
+__device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_scf_opts& o, int k,
+                                    double loose, const glbf* vin) {
+    const int n = M.n, m = n - k;
+    if (k <= 0 || m <= 0 || 2 * k > n) return 0;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    glbf* Ct = M.Ct;
+    glbf* Cv = M.Ct + (size_t)k * n;
+    glbf* Kb = M.P;  // k x m, leading dimension m
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass > 0) fock_build(M, vin);
+        // T = C F (all rows, for the diagonal); A_ov = T_occ C_vir^T
+        gemm<false, false>(n, n, n, 1.0, Ct, n, M.F, n, 0.0, M.T, n, sh.big);
+        gemm<false, true>(k, m, n, 1.0, M.T, n, Cv, n, 0.0, Kb, m, sh.big);
+        for (int i = w; i < n; i += NWAVE) {
+            double d = 0.0;
+            for (int c = lane; c < n; c += WAVE) d = fma(M.T[(size_t)i * n + c], Ct[(size_t)i * n + c], d);
+            d = wave_sum(d);
+            if (lane == 0) sh.ev[i] = d;
+        }
+        __syncthreads();
+        double dmax = 0.0;
+        for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, fabs(sh.ev[i]));
+        dmax = wg_max(dmax, sh.red);
+        const double tol = fmax(o.jacobi_tol * fmax(dmax, 1.0), loose);
+        double aov = 0.0, kmax = 0.0, dmin = INFINITY;
+        for (int i = w; i < k; i += NWAVE) {
+            const double ei = sh.ev[i];
+            for (int a = lane; a < m; a += WAVE) {
+                const double v = Kb[(size_t)i * m + a];
+                const double den = sh.ev[k + a] - ei;
+                const double amp = v / den;
+                aov = fmax(aov, fabs(v));
+                dmin = fmin(dmin, den);
+                kmax = fmax(kmax, fabs(amp));
+                Kb[(size_t)i * m + a] = amp;
+            }
+        }
+        aov = wg_max(aov, sh.red);
+        kmax = wg_max(kmax, sh.red);
+        dmin = -wg_max(-dmin, sh.red);
+        if (aov <= tol) return 1;  // decoupled: orbitals stay
+        const int fail = pass == 0 ? 0 : -1;
+        if (!(dmin > 0.0) || !(kmax <= o.pd_kmax)) return fail;
+        glbf* X = M.T;
+        glbf* Z = M.T + (size_t)k * k;
+        gemm<false, true>(k, k, m, 1.0, Kb, m, Kb, m, 0.0, X, k, sh.big);
+        gemm<false, false>(k, n, m, 1.0, Kb, m, Cv, n, 0.0, Z, n, sh.big);
+        double xf = 0.0;
+        for (int e = threadIdx.x; e < k * k; e += NT) xf = fma(X[e], X[e], xf);
+        xf = sqrt(wg_sum(xf, sh.red));
+        // highest power J: first omitted cosine term ||X||^(J+1) / (2J+2)! < 1e-17
+        int J = 1;
+        double term = xf * xf / 24.0;
+        while (term > 1e-17) {
+            ++J;
+            term *= xf / ((2.0 * J + 1.0) * (2.0 * J + 2.0));
+            if (J > 8) return fail;  // F and Ct untouched in this pass
+        }
+        double cc[9], ss[9], ph[9];
+        {
+            double f = 1.0;  // (2j)!
+            for (int j = 0; j <= 8; ++j) {
+                if (j > 0) f *= (2.0 * j - 1.0) * (2.0 * j);
+                const double sg = (j & 1) ? -1.0 : 1.0;
+                cc[j] = sg / f;
+                ss[j] = sg / (f * (2.0 * j + 1.0));
+                ph[j] = -sg / (f * (2.0 * j + 1.0) * (2.0 * j + 2.0));
+            }
+        }
+        glbf* acc = M.F;
+        glbf* tmp = M.F + (size_t)k * n;
+        // R = sinc(X) C_occ + phi(X) Z  ->  acc
+        for (int i = w; i < k; i += NWAVE)
+            for (int c = lane; c < n; c += WAVE) {
+                const size_t e = (size_t)i * n + c;
+                acc[e] = ss[J] * Ct[e] + ph[J] * Z[e];
+            }
+        __syncthreads();
+        for (int j = J - 1; j >= 0; --j) {
+            gemm<false, false>(k, n, k, 1.0, X, k, acc, n, 0.0, tmp, n, sh.big);
+            for (int i = w; i < k; i += NWAVE)
+                for (int c = lane; c < n; c += WAVE) {
+                    const size_t e = (size_t)i * n + c;
+                    acc[e] = tmp[e] + ss[j] * Ct[e] + ph[j] * Z[e];
+                }
+            __syncthreads();
+        }
+        // C_vir += K^T R
+        gemm<true, false>(m, n, k, 1.0, Kb, m, acc, n, 1.0, Cv, n, sh.big);
+        // C_occ <- cos(X) C_occ - sinc(X) Z
+        for (int i = w; i < k; i += NWAVE)
+            for (int c = lane; c < n; c += WAVE) {
+                const size_t e = (size_t)i * n + c;
+                acc[e] = cc[J] * Ct[e] - ss[J] * Z[e];
+            }
+        __syncthreads();
+        for (int j = J - 1; j >= 0; --j) {
+            gemm<false, false>(k, n, k, 1.0, X, k, acc, n, 0.0, tmp, n, sh.big);
+            for (int i = w; i < k; i += NWAVE)
+                for (int c = lane; c < n; c += WAVE) {
+                    const size_t e = (size_t)i * n + c;
+                    acc[e] = tmp[e] + cc[j] * Ct[e] - ss[j] * Z[e];
+                }
+            __syncthreads();
+        }
+        for (int i = w; i < k; i += NWAVE)
+            for (int c = lane; c < n; c += WAVE) Ct[(size_t)i * n + c] = acc[(size_t)i * n + c];
+        __syncthreads();
+        if (kmax * aov <= tol) return 1;  // second-order remainder below the tolerance
+    }
+    return 1;
+}
+
+// ---------------------------------------------------------------------------
 // one SCF map v_in -> v_out (scf/base.py:651-675 iterate_potential);
 // leaves P, q, e_orb, sorted e/occupations and shell/atom charges behind.
 // ---------------------------------------------------------------------------
@@ -298,23 +452,28 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     const size_t nn = (size_t)n * n;
     // Fock build (scf/base.py:817-859)
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    for (int r = w; r < n; r += NWAVE) {
-        const double vr = vin[r];
-        for (int c = lane; c < n; c += WAVE) {
-            const size_t e = (size_t)r * n + c;
-            M.F[e] = M.H0[e] - 0.5 * M.S[e] * (vr + vin[c]);
-        }
-    }
-    __syncthreads();
-    // T = F C ; A = C^T T  -> F
+    fock_build(M, vin);
     double t0 = now_us();
-    gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
-    gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
-    double t1 = now_us();
-    const int sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi);
-    double t2 = now_us();
-    sh.tacc[1] += t1 - t0;
-    sh.tacc[2] += t2 - t1;
+    int sw = 0;
+    bool pd = false;
+    if (cb_lo >= 0 && cb_lo == cb_hi && o.pd_kmax > 0.0) {
+        const int r = pd_step(M, sh, o, cb_lo, loose, vin);
+        pd = r > 0;
+        if (r < 0) fock_build(M, vin);  // F served as scratch
+    }
+    if (pd) {
+        sh.tacc[2] += now_us() - t0;
+        sh.tacc[6] += 1.0;
+    } else {
+        // T = F C ; A = C^T T  -> F
+        gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
+        gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
+        double t1 = now_us();
+        sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi);
+        double t2 = now_us();
+        sh.tacc[1] += t1 - t0;
+        sh.tacc[2] += t2 - t1;
+    }
     if (sw < 0) *status |= XTB_MOL_JACOBI;
     // sort eigenvalues
     for (int j = threadIdx.x; j < sh.npow; j += NT) {
@@ -671,7 +830,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_T_TOTAL] = now_us() - tstart;
         out[XTB_OUT_T_BSUB] = tacc[4];
         out[XTB_OUT_T_BROW] = tacc[5];
-        out[XTB_OUT_T_BCOL] = tacc[6];
+        out[XTB_OUT_PD_STEPS] = tacc[6];
         out[XTB_OUT_SWEEPS_FIRST] = (double)sw_first;
         out[XTB_OUT_SWEEPS_LAST] = (double)sw_last;
     }

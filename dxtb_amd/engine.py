@@ -194,6 +194,9 @@ class ScfOptions:
     # intermediate SCF steps decouple occupied from virtual orbitals only
     # (all that the density needs) when the frontier gaps exceed this many kT
     bd_gap_kt: float = 60.0
+    # closed shells: decouple by one perturbative occupied-virtual rotation
+    # (matrix-core GEMMs) when its largest amplitude is at most this
+    pd_kmax: float = 0.05
 
     def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
@@ -205,6 +208,7 @@ class ScfOptions:
             x_atol=self.x_atol, x_atol_max=self.x_atol_max, kt=self.fermi_etemp * C.KELVIN2AU,
             damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
             fermi_thresh=thr, jacobi_tol=self.jacobi_tol, bd_gap_kt=self.bd_gap_kt,
+            pd_kmax=self.pd_kmax,
         )
 
 
@@ -218,8 +222,14 @@ class Pipeline:
         self.lib = _lib.load()
         dev = self.device
         self.d = {}
+        # pinned staging + asynchronous copies: a pageable copy would block
+        # the host until the stream (or a busy stream sharing its hardware
+        # queue) reaches it.  The staging tensors live as long as the pipeline.
+        self._staging = []
         for k, a in layout.arrays.items():
-            self.d[k] = torch.as_tensor(np.ascontiguousarray(a), device=dev)
+            h = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
+            self._staging.append(h)
+            self.d[k] = h.to(dev, non_blocking=True)
         z = lambda k: torch.empty(k, dtype=torch.float64, device=dev)
         L = layout
         nm, nv = L.nmat_total, L.nvec_total

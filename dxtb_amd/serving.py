@@ -12,7 +12,9 @@ Requests (one molecule each, arbitrary size) arrive over time.  The server
   analytic forces) for the micro-batch on its own HIP stream, so launches
   overlap on the device: one workgroup per molecule, and workgroups of a
   newer launch fill compute units that molecules of an older launch have
-  released by converging;
+  released by converging.  Three streams by default: HIP maps streams onto
+  the process's hardware queues (four, one of them taken by the default
+  stream), and two streams on one queue run their work one after the other;
 * keeps each molecule's SCF state (potential, Anderson history, orbitals,
   density) resident in that launch's device buffers until its results are
   copied out, then frees the buffers and backfills with waiting requests.
@@ -75,7 +77,7 @@ def orbital_count(numbers: np.ndarray) -> int:
 
 
 class RaggedServer:
-    def __init__(self, device="cuda:0", opts: ScfOptions | None = None, streams: int = 4,
+    def __init__(self, device="cuda:0", opts: ScfOptions | None = None, streams: int = 3,
                  edges=DEFAULT_EDGES, max_batch=DEFAULT_MAXB, hbm_budget_gb: float = 64.0,
                  elements=(1, 6, 7, 8), forces: bool = True, max_wait_s: float = 0.002,
                  timing: bool = False):

@@ -82,13 +82,13 @@ extern "C" {
 #define XTB_OUT_EREP 9      /* written by xtb_repulsion */
 #define XTB_OUT_SWEEPS 10   /* total Jacobi sweeps spent */
 /* stage timers of the SCF kernel, microseconds (device wall clock) */
-#define XTB_OUT_T_ORTHO 11  /* orthogonaliser (eigensolve of S) */
+#define XTB_OUT_T_ORTHO 11  /* orthogonaliser S^-1/2 (Newton-Schulz) */
 #define XTB_OUT_T_GEMM 12   /* Fock transform, density and W products */
-#define XTB_OUT_T_EIGEN 13  /* Jacobi sweeps of the SCF steps */
+#define XTB_OUT_T_EIGEN 13  /* eigensolves / decoupling of the SCF steps */
 #define XTB_OUT_T_TOTAL 14  /* whole molecule */
 #define XTB_OUT_T_BSUB 15   /* block Jacobi: subproblem solves */
 #define XTB_OUT_T_BROW 16   /* block Jacobi: row-panel updates (A and vectors) */
-#define XTB_OUT_T_BCOL 17   /* block Jacobi: column-panel updates */
+#define XTB_OUT_PD_STEPS 17 /* SCF steps decoupled by the perturbative rotation */
 #define XTB_OUT_SWEEPS_FIRST 18  /* Jacobi sweeps of the first SCF step */
 #define XTB_OUT_SWEEPS_LAST 19   /* Jacobi sweeps of the final (exact) step */
 
@@ -180,6 +180,9 @@ typedef struct xtb_scf_opts {
     double bd_gap_kt;        /* intermediate SCF steps only decouple occupied from
                                 virtual orbitals when the previous step's frontier
                                 gaps are >= this many kT (0: always full eigensolve) */
+    double pd_kmax;          /* closed shells: decouple by one perturbative
+                                occupied-virtual rotation when its largest
+                                amplitude is <= this (0: Jacobi only) */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */

@@ -129,21 +129,28 @@ def test_inexact_eigensolves_reach_the_same_fixed_point(fx):
 
 def test_block_diagonal_steps_reach_the_same_fixed_point(fx):
     """Intermediate SCF steps of gapped molecules only decouple occupied from
-    virtual orbitals (the density of 0/1 occupations needs nothing more); the
-    final step is a full eigensolve.  Results must equal those of full
+    virtual orbitals (the density of 0/1 occupations needs nothing more), by a
+    perturbative rotation or by Jacobi on the inter-class pairs; the final
+    step is a full eigensolve.  Results must equal those of full
     eigensolves in every step, to the parity tolerances.  Covers the packed
     scalar Jacobi (n < 190) and block Jacobi (n > 190) paths."""
     rng = np.random.default_rng(8)
     mols = [mol(fx, "lys_xao"), W.chain(rng, 12), W.chain(rng, 35), W.water_cluster(rng, 30)]
     print("case: occupied/virtual decoupling vs full eigensolves", flush=True)
-    a = run_native(mols, forces=True)
     b = run_native(mols, forces=True, bd_gap_kt=0.0)
-    assert a.records()[:, 6].abs().max().item() == 0
-    assert np.abs(a.energies().cpu().numpy() - b.energies().cpu().numpy()).max() < 1e-9
-    assert np.abs(a.gradient().cpu().numpy() - b.gradient().cpu().numpy()).max() < 1e-8
-    for k in range(len(mols)):
-        qa, qb = a.orbital_charges(k).cpu().numpy(), b.orbital_charges(k).cpu().numpy()
-        assert np.abs(qa - qb).max() < 1e-8
-        ea, eb = a.emo[int(a.L.vec0[k]):int(a.L.vec0[k]) + int(a.L.n[k])], \
-            b.emo[int(b.L.vec0[k]):int(b.L.vec0[k]) + int(b.L.n[k])]
-        assert (ea - eb).abs().max().item() < 1e-8
+    for opt in ({}, {"pd_kmax": 0.0}):
+        # default: perturbative rotations where they apply, else Jacobi on the
+        # inter-class pairs; pd_kmax=0: Jacobi on the inter-class pairs only
+        a = run_native(mols, forces=True, **opt)
+        rec = a.records().cpu().numpy()
+        assert np.abs(rec[:, 6]).max() == 0
+        if not opt:
+            assert rec[:, 17].max() > 0  # the perturbative path ran
+        assert np.abs(a.energies().cpu().numpy() - b.energies().cpu().numpy()).max() < 1e-9
+        assert np.abs(a.gradient().cpu().numpy() - b.gradient().cpu().numpy()).max() < 1e-8
+        for k in range(len(mols)):
+            qa, qb = a.orbital_charges(k).cpu().numpy(), b.orbital_charges(k).cpu().numpy()
+            assert np.abs(qa - qb).max() < 1e-8
+            ea, eb = a.emo[int(a.L.vec0[k]):int(a.L.vec0[k]) + int(a.L.n[k])], \
+                b.emo[int(b.L.vec0[k]):int(b.L.vec0[k]) + int(b.L.n[k])]
+            assert (ea - eb).abs().max().item() < 1e-8
