@@ -78,7 +78,7 @@ __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 // to mol.Ct.  Returns Jacobi sweeps (-1 on failure).
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_sweeps, double rel_tol,
-                                     double loose, int inner, int cb_lo, int cb_hi) {
+                                     double loose, int inner, int cb_lo, int cb_hi, bool intra) {
     const int n = M.n;
     // scale for the absolute tolerance
     double dmax = 0.0;
@@ -97,8 +97,8 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
             for (int idx = threadIdx.x; idx < n * n; idx += NT) cbuf[idx] = M.Ct[idx];
         }
         __syncthreads();
-        if (cl) sweeps = wg_jacobi(A, n, RowsLds{cbuf, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi);
-        else sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi);
+        if (cl) sweeps = wg_jacobi(A, n, RowsLds{cbuf, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi, intra);
+        else sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi, intra);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = A.at(i, i);
         if (cl) {
             for (int idx = threadIdx.x; idx < n * n; idx += NT) M.Ct[idx] = cbuf[idx];
@@ -106,11 +106,11 @@ __device__ __noinline__ int eig_step(const Mol& M, const Shared& sh, int max_swe
     } else if (sh.big_cap >= NBLOCK_LDS) {
         // block Jacobi; T is free scratch here and holds the subproblem rotations
         sweeps = wg_block_jacobi(M.F, n, RowsGlobal{M.Ct, n}, M.T, max_sweeps, tol, sh.big, sh.red,
-                                 sh.tacc, inner, cb_lo, cb_hi);
+                                 sh.tacc, inner, cb_lo, cb_hi, intra);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = M.F[(size_t)i * n + i];
     } else {
         SymFull A{M.F, n};
-        sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi);
+        sweeps = wg_jacobi(A, n, RowsGlobal{M.Ct, n}, max_sweeps, tol, sh.cs, sh.red, cb_lo, cb_hi, intra);
         for (int i = threadIdx.x; i < n; i += NT) sh.ev[i] = A.at(i, i);
     }
     __syncthreads();
@@ -431,7 +431,7 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
                         const xtb_params& par, const xtb_scf_opts& o, const glbf* vin,
                         glbf* vout, int* status, double* ef, double loose, bool allow_bd,
-                        bool sort_rows) {
+                        bool sort_rows, bool final_step = false) {
     const int n = M.n;
     // Occupation classes for this step's eigensolve.  With integer alpha/beta
     // counts and frontier gaps of at least bd_gap_kt x kT in the previous
@@ -440,7 +440,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     // removed: that fixes the density of 0/1 occupations, whose Fermi tails
     // at such gaps are below 1/(1 + e^(bd_gap_kt/2)).
     int cb_lo = -1, cb_hi = -1;
-    if (allow_bd && o.bd_gap_kt > 0.0 && !o.padded) {
+    if ((allow_bd || final_step) && o.bd_gap_kt > 0.0 && !o.padded) {
         const double na = M.nel_a, nb = M.nel_b;
         const int ia = (int)rint(na), ib = (int)rint(nb);
         bool ok = fabs(na - ia) < 1e-12 && fabs(nb - ib) < 1e-12 && ib <= ia;
@@ -455,8 +455,18 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     fock_build(M, vin);
     double t0 = now_us();
     int sw = 0;
-    bool pd = false;
-    if (cb_lo >= 0 && cb_lo == cb_hi && o.pd_kmax > 0.0) {
+    bool pd = false, intra = false;
+    if (final_step) {
+        // Final (exact) step of a gapped closed shell: remove the last
+        // occupied-virtual coupling perturbatively (it is second order after
+        // that), then diagonalise inside the two classes only.
+        if (cb_lo >= 0 && cb_lo == cb_hi && o.pd_kmax > 0.0) {
+            const int r = pd_step(M, sh, o, cb_lo, 0.0, vin);
+            intra = r > 0;
+            if (r != 0) fock_build(M, vin);  // F may have served as scratch
+        }
+        if (!intra) cb_lo = cb_hi = -1;
+    } else if (cb_lo >= 0 && cb_lo == cb_hi && o.pd_kmax > 0.0) {
         const int r = pd_step(M, sh, o, cb_lo, loose, vin);
         pd = r > 0;
         if (r < 0) fock_build(M, vin);  // F served as scratch
@@ -469,7 +479,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
         gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
         double t1 = now_us();
-        sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi);
+        sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi, intra);
         double t2 = now_us();
         sh.tacc[1] += t1 - t0;
         sh.tacc[2] += t2 - t1;
@@ -759,7 +769,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     // step, from which every reported quantity comes, to full tolerance.
     double resid = 1e-2;
     const int sw_first = scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef,
-                                  o.eig_inexact > 0.0 ? 1e-4 : 0.0, false, true);
+                                  o.eig_inexact > 0.0 ? o.first_tol : 0.0, false, true);
     sweeps += sw_first;
     ++iters;
     if (o.mix_guess) {
@@ -780,7 +790,8 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     if (o.maxiter == 0) converged = true;
     // converged_to_charges (scf/base.py:472-513): density of the last SCF
     // output potential; the potential of its charges goes to st.pot
-    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false, false);
+    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false, false,
+                                 o.eig_inexact > 0.0);
     sweeps += sw_last;
 
     // energies (scf/base.py:514-623)

@@ -254,15 +254,20 @@ __device__ inline void rr_pair(int r, int k, int n2, int& p, int& q) {
 // when max_sweeps were spent without reaching `tol` (absolute).
 //
 // Occupation classes (cb_lo >= 0): index i has class (i < cb_lo) + (i < cb_hi)
-// (doubly, singly, un-occupied for sorted orbitals).  Only pairs of different
-// classes are rotated and tested, which decouples the classes without
-// diagonalising inside them; the density of integer occupations needs no more.
+// (doubly, singly, un-occupied for sorted orbitals).  Mode 1 rotates and
+// tests only pairs of different classes, which decouples the classes without
+// diagonalising inside them (the density of integer occupations needs no
+// more); mode 2 only pairs of the same class, which diagonalises inside
+// classes that are already decoupled.  Mode 0 (cb_lo < 0): every pair.
 __device__ inline int occ_class(int i, int cb_lo, int cb_hi) { return (i < cb_lo) + (i < cb_hi); }
+__device__ inline bool pair_in_mode(int mode, int ci, int cj) {
+    return mode == 0 || ((ci != cj) == (mode == 1));
+}
 
 template <class Sym, class CAcc>
 __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf* cs,
-                         ldsf* red, int cb_lo = -1, int cb_hi = -1) {
-    const bool bd = cb_lo >= 0;
+                         ldsf* red, int cb_lo = -1, int cb_hi = -1, bool intra = false) {
+    const int mode = cb_lo < 0 ? 0 : (intra ? 2 : 1);
     const int n2 = n + (n & 1);
     const int npair = n2 / 2;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
@@ -276,7 +281,7 @@ __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf
         for (int i = w; i < n; i += NWAVE) {
             const int ci = occ_class(i, cb_lo, cb_hi);
             for (int j = i + 1 + lane; j < n; j += WAVE)
-                if (!bd || occ_class(j, cb_lo, cb_hi) != ci) off = fmax(off, fabs(A.at(i, j)));
+                if (pair_in_mode(mode, ci, occ_class(j, cb_lo, cb_hi))) off = fmax(off, fabs(A.at(i, j)));
         }
         off = wg_max(off, red);
         if (off <= tol) return sweep;
@@ -288,7 +293,7 @@ __device__ int wg_jacobi(Sym A, int n, CAcc Ct, int max_sweeps, double tol, ldsf
                 int p, q;
                 rr_pair(r, k, n2, p, q);
                 double c = 1.0, s = 0.0;
-                if (p < n && q < n && (!bd || occ_class(p, cb_lo, cb_hi) != occ_class(q, cb_lo, cb_hi))) {
+                if (p < n && q < n && pair_in_mode(mode, occ_class(p, cb_lo, cb_hi), occ_class(q, cb_lo, cb_hi))) {
                     const double apq = A.at(p, q);
                     if (fabs(apq) > skip) {
                         const double app = A.at(p, p), aqq = A.at(q, q);
@@ -434,10 +439,10 @@ __device__ inline void build_schedules(ldsi* tab) {
 // of the two-sided update, and 8 (pair, row) elements of Qt.  Items touch
 // disjoint elements, so no ordering among them is needed.
 __device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const ldsi* sched,
-                                            double tol, int max_sweeps, bool bd, unsigned lo_mask,
+                                            double tol, int max_sweeps, int mode, unsigned lo_mask,
                                             unsigned hi_mask) {
     const int lane = threadIdx.x % WAVE;
-    // occupation class of a slot (when bd, only inter-class pairs rotate)
+    // occupation class of a slot (pairs are selected by mode as in wg_jacobi)
     auto scls = [&](int i) { return (int)((lo_mask >> i) & 1u) + (int)((hi_mask >> i) & 1u); };
     const ldsi* rrtab = sched;
     const ldsi* blktab = sched + (JS - 1) * NPAIR32;
@@ -460,7 +465,7 @@ __device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const l
         double off = 0.0;
         for (int e = lane; e < JS * JS; e += WAVE) {
             const int i = e / JS, j = e % JS;
-            if (j > i && (!bd || scls(i) != scls(j))) off = fmax(off, fabs(pk(M, i, j)));
+            if (j > i && pair_in_mode(mode, scls(i), scls(j))) off = fmax(off, fabs(pk(M, i, j)));
         }
         off = wave_max(off);
         if (off <= tol) break;
@@ -472,7 +477,7 @@ __device__ __noinline__ int wave_jacobi_sub(ldsf* M, ldsf* Qt, ldsf* cs, const l
                 const int p = pq & 255, q = pq >> 8;
                 double c = 1.0, s = 0.0;
                 const double apq = pk(M, p, q), app = pk(M, p, p), aqq = pk(M, q, q);
-                if (fabs(apq) > skip && (!bd || scls(p) != scls(q))) {
+                if (fabs(apq) > skip && pair_in_mode(mode, scls(p), scls(q))) {
                     const double theta = (aqq - app) / (2.0 * apq);
                     double t = 1.0 / (fabs(theta) + sqrt(fma(theta, theta, 1.0)));
                     if (theta < 0.0) t = -t;
@@ -692,8 +697,11 @@ __device__ __noinline__ void rows_update(glbf* X, int n, size_t ld, int ncol, co
 template <class CAcc>
 __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf, int max_sweeps, double tol,
                                             ldsf* lds, ldsf* red, double* tacc, int inner,
-                                            int cb_lo, int cb_hi) {
-    const bool bd = cb_lo >= 0;
+                                            int cb_lo, int cb_hi, bool intra) {
+    const int mode = cb_lo < 0 ? 0 : (intra ? 2 : 1);
+    // mode 2 skips pairs of pure blocks of different classes; every block then
+    // still meets a block of its own class if each class spans two blocks
+    const bool skip_cross = mode == 2 && cb_lo >= 2 * JB && n - cb_hi >= 2 * JB;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     ldsf* M = lds + w * JWAVE_LDS;
     ldsf* Q = M + JPK;
@@ -716,7 +724,7 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
         for (int i = w; i < n; i += NWAVE) {
             const int ci = occ_class(i, cb_lo, cb_hi);
             for (int j = i + 1 + lane; j < n; j += WAVE)
-                if (!bd || occ_class(j, cb_lo, cb_hi) != ci) off = fmax(off, fabs(A[(size_t)i * n + j]));
+                if (pair_in_mode(mode, ci, occ_class(j, cb_lo, cb_hi))) off = fmax(off, fabs(A[(size_t)i * n + j]));
         }
         off = wg_max(off, red);
         if (off <= tol) return sweep;
@@ -729,18 +737,28 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
                 int I, J;
                 rr_pair(r, k, nb2, I, J);
                 unsigned lo_mask = 0u, hi_mask = 0u;
-                if (bd) {
-                    // slot classes; a pair whose valid slots share one class
-                    // has nothing to decouple
+                if (mode != 0) {
+                    // slot classes (slots 0..15 block I, 16..31 block J)
                     const int g = slot_index(I, J, lane % JS, n);
-                    const unsigned long long v = __ballot(lane < JS && g >= 0);
-                    const unsigned long long lo = __ballot(lane < JS && g >= 0 && g < cb_lo);
-                    const unsigned long long hi = __ballot(lane < JS && g >= 0 && g < cb_hi);
-                    const unsigned vm = (unsigned)v;
-                    lo_mask = (unsigned)lo;
-                    hi_mask = (unsigned)hi;
-                    const bool pure = (lo_mask == 0u || lo_mask == vm) && (hi_mask == 0u || hi_mask == vm);
-                    if (pure) {
+                    const unsigned vm = (unsigned)__ballot(lane < JS && g >= 0);
+                    lo_mask = (unsigned)__ballot(lane < JS && g >= 0 && g < cb_lo);
+                    hi_mask = (unsigned)__ballot(lane < JS && g >= 0 && g < cb_hi);
+                    bool skip;
+                    if (mode == 1) {
+                        // a pair whose valid slots share one class has nothing to decouple
+                        skip = (lo_mask == 0u || lo_mask == vm) && (hi_mask == 0u || hi_mask == vm);
+                    } else {
+                        // two non-empty pure blocks of different classes: no
+                        // same-class pair spans them
+                        const unsigned vI = vm & 0xFFFFu, vJ = vm >> 16;
+                        const unsigned lI = lo_mask & 0xFFFFu, lJ = lo_mask >> 16;
+                        const unsigned hI = hi_mask & 0xFFFFu, hJ = hi_mask >> 16;
+                        const bool pI = (lI == 0u || lI == vI) && (hI == 0u || hI == vI);
+                        const bool pJ = (lJ == 0u || lJ == vJ) && (hJ == 0u || hJ == vJ);
+                        const int cI = (lI != 0u) + (hI != 0u), cJ = (lJ != 0u) + (hJ != 0u);
+                        skip = skip_cross && vI != 0u && vJ != 0u && pI && pJ && cI != cJ;
+                    }
+                    if (skip) {
                         if (lane == 0) ident[k] = 1;
                         continue;
                     }
@@ -756,7 +774,7 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
                     }
                 }
                 wave_sync();
-                const int rot = wave_jacobi_sub(M, Q, cs, sched, tol, inner, bd, lo_mask, hi_mask);
+                const int rot = wave_jacobi_sub(M, Q, cs, sched, tol, inner, mode, lo_mask, hi_mask);
                 glbf* qg = Qbuf + (size_t)k * JS * JS;
                 if (lane == 0) ident[k] = rot ? 0 : 1;
                 if (rot) {

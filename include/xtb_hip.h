@@ -183,6 +183,8 @@ typedef struct xtb_scf_opts {
     double pd_kmax;          /* closed shells: decouple by one perturbative
                                 occupied-virtual rotation when its largest
                                 amplitude is <= this (0: Jacobi only) */
+    double first_tol;        /* eigensolve tolerance of the first SCF step
+                                (used when eig_inexact > 0) */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */
