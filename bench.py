@@ -136,6 +136,21 @@ def roofline_pass(stream, start: int, count: int, device, opts) -> dict:
             "flops_per_launch": float(fl.mean())}
 
 
+def measured_traffic(chunk: int, warmup: int) -> float | None:
+    """HBM bytes per k_scf launch of this roofline pass, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) summarised in
+    profiles/k_scf_traffic.json; None when the workload differs."""
+    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_scf_traffic.json")
+    try:
+        with open(f) as fh:
+            t = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if t.get("chunk") != chunk or t.get("warmup") != warmup or t.get("seed") != 4:
+        return None
+    return float(t["traffic_bytes_per_launch"])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -266,7 +281,7 @@ def main() -> int:
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                     "traffic": None, "kernel": "xtb::k_scf",
+                     "traffic": measured_traffic(a.chunk, a.warmup), "kernel": "xtb::k_scf",
                      "launches": rp["launches"], "mean_launch_ms": round(rp["mean_launch_ms"], 3),
                      "flops_per_launch": rp["flops_per_launch"],
                      "measured_peaks": peaks},

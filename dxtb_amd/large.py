@@ -149,7 +149,10 @@ class LargeSystem:
         import ctypes as ct
 
         L, pipe, dev = self.layout, self.pipe, self.device
-        pos = torch.as_tensor(np.asarray(positions), dtype=torch.float64, device=dev).contiguous()
+        if isinstance(positions, torch.Tensor):
+            pos = positions.detach().to(device=dev, dtype=torch.float64).contiguous()
+        else:
+            pos = torch.as_tensor(np.asarray(positions), dtype=torch.float64, device=dev).contiguous()
         pipe.positions = pos
         s = ct.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         lib, b, p = pipe.lib, ct.byref(pipe.batch), ct.byref(self.dt.s)
@@ -249,7 +252,7 @@ class LargeSystem:
         erep = float(pipe.out[_lib.OUT_EREP])
         res = {"energy": e_h0 + e_es2 + e_es3 + erep, "scf": e_h0 + e_es2 + e_es3,
                "repulsion": erep, "iterations": iters, "converged": converged,
-               "charges_orbital": q, "gap": gap}
+               "charges_orbital": q, "charges_atom": qat, "gap": gap}
         if forces:
             W = 0.5 * (P @ F @ P) if na == nb else None
             if W is None:

@@ -40,3 +40,30 @@ def test_large_path_matches_oracle(fx, case):
     assert np.abs(r["gradient"].cpu().numpy() - o["gradient"].numpy()).max() < 1e-8
     k = run_native([(z, p)], forces=True)
     assert abs(k.energies().item() - r["energy"]) < 1e-9
+
+
+def test_config3_water600_vs_oracle_fixture():
+    """600 H2O (1800 atoms, 4800 orbitals) through the calculator, which sends
+    a single molecule of this size to the whole-GPU path; the reference
+    values come from the oracle (tests/golden/make_config3.py, CPU FP64,
+    x_atol 1e-10).  Targets: |dE| <= 1e-8 Eh, atomic charges <= 1e-7 e,
+    forces <= 1e-7 Eh/bohr."""
+    import os
+
+    from dxtb_amd import GFN1Calculator
+
+    ref = np.load(os.path.join(os.path.dirname(__file__), "golden", "config3_oracle.npz"))
+    z = torch.as_tensor(ref["numbers"], device="cuda:0")
+    pos = torch.as_tensor(ref["positions"], dtype=torch.float64, device="cuda:0")
+    print("case: config 3 (600 H2O) vs oracle fixture", flush=True)
+    calc = GFN1Calculator(z, dtype=torch.float64, device="cuda:0",
+                          opts={"x_atol": 1e-10, "x_atol_max": 1e-10, "maxiter": 300,
+                                "exclude": ["disp"]})
+    e = calc.get_energy(pos)
+    assert abs(float(e) - float(ref["energy"])) < 1e-8
+    f = calc.forces_analytical(pos)
+    assert np.abs(-f.cpu().numpy() - ref["gradient"]).max() < 1e-7
+    q = calc.get_charges(pos)
+    assert q.shape == (4800,) and abs(float(q.sum())) < 1e-8
+    qat = calc._run(pos, forces=False)["large"]["charges_atom"]
+    assert np.abs(qat.cpu().numpy() - ref["charges_atom"]).max() < 1e-7
