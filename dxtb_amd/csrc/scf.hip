@@ -319,17 +319,26 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
     glbf* Ct = M.Ct;
     glbf* Cv = M.Ct + (size_t)k * n;
     glbf* Kb = M.P;  // k x m, leading dimension m
+    // Virtual orbital energies only scale the amplitudes: an error of one
+    // SCF step in them leaves a second-order coupling, so every other
+    // perturbative step reuses the previous step's values (sh.skey, in the
+    // rows' order) and forms only the occupied rows of C F.
+    // sh.tacc[7] = 1 marks the virtual energies as fresh.
+    const bool refresh = sh.tacc[7] == 0.0;
+    sh.tacc[7] = refresh ? 1.0 : 0.0;
+    const int rows = refresh ? n : k;
     for (int pass = 0; pass < 2; ++pass) {
         if (pass > 0) fock_build(M, vin);
-        // T = C F (all rows, for the diagonal); A_ov = T_occ C_vir^T
-        gemm<false, false>(n, n, n, 1.0, Ct, n, M.F, n, 0.0, M.T, n, sh.big);
+        // T = C F (rows: all, or the occupied ones); A_ov = T_occ C_vir^T
+        gemm<false, false>(rows, n, n, 1.0, Ct, n, M.F, n, 0.0, M.T, n, sh.big);
         gemm<false, true>(k, m, n, 1.0, M.T, n, Cv, n, 0.0, Kb, m, sh.big);
-        for (int i = w; i < n; i += NWAVE) {
+        for (int i = w; i < rows; i += NWAVE) {
             double d = 0.0;
             for (int c = lane; c < n; c += WAVE) d = fma(M.T[(size_t)i * n + c], Ct[(size_t)i * n + c], d);
             d = wave_sum(d);
             if (lane == 0) sh.ev[i] = d;
         }
+        for (int a = rows + threadIdx.x; a < n; a += NT) sh.ev[a] = sh.skey[a];
         __syncthreads();
         double dmax = 0.0;
         for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, fabs(sh.ev[i]));
@@ -480,6 +489,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
         double t1 = now_us();
         sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi, intra);
+        sh.tacc[7] = 1.0;  // eigenvalues fresh for the next perturbative step
         double t2 = now_us();
         sh.tacc[1] += t1 - t0;
         sh.tacc[2] += t2 - t1;

@@ -127,6 +127,12 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
     const int tm = (M + TILE - 1) / TILE, tn = (N + TILE - 1) / TILE;
     for (int tile = 0; tile < tm * tn; ++tile) {
         const int m0 = (tile / tn) * TILE, n0 = (tile % tn) * TILE;
+        // 16x16 fragments of this wave that hold any output (the last tile
+        // row/column is partial): the others skip their MFMAs (wave-uniform)
+        const int rrem = M - (m0 + wm * (TILE / 2)), crem = N - (n0 + wn * (TILE / 4));
+        const int nfm = rrem <= 0 ? 0 : (rrem >= FM * 16 ? FM : (rrem + 15) / 16);
+        const int nfn = crem <= 0 ? 0 : (crem >= FN * 16 ? FN : (crem + 15) / 16);
+        const bool full = nfm == FM && nfn == FN;
         v4d acc[FM][FN];
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -178,10 +184,18 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                 for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDT + wm * (TILE / 2) + i * 16 + li];
 #pragma unroll
                 for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDT + wn * (TILE / 4) + j * 16 + li];
+                if (full) {
 #pragma unroll
-                for (int i = 0; i < FM; ++i)
+                    for (int i = 0; i < FM; ++i)
 #pragma unroll
-                    for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+                        for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+                } else {
+#pragma unroll
+                    for (int i = 0; i < FM; ++i)
+#pragma unroll
+                        for (int j = 0; j < FN; ++j)
+                            if (i < nfm && j < nfn) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+                }
             }
             __syncthreads();
         }
