@@ -109,26 +109,52 @@ __device__ inline v4d v4zero() {
 }
 
 // C = alpha op(A) op(B) + beta C on matrix cores.  Workgroup tile TILE x TILE
-// (TILE = 64 or 128), 8 waves as 2 (rows) x 4 (cols), k panels of 16 staged in
-// LDS as As[k][m], Bs[k][n].  `lds` holds >= 2*16*(TILE+4) doubles.
-template <bool TA, bool TB, int TILE>
+// (TILE = 64 or 128), 8 waves as 2 (rows) x 4 (cols), k panels of KP staged
+// in LDS as As[k][m], Bs[k][n] (row pitch TILE+1: conflict-free transposed
+// stores), two stages: the next panel is fetched into registers before the
+// current panel's MFMAs and stored to the other stage after them, so one
+// barrier per panel separates the stages.  `lds` holds >= 4*KP*(TILE+1)
+// doubles.  Each global load instruction covers contiguous memory: lanes run
+// along m (or n) when the operand is stored that way, along k otherwise.
+template <bool TA, bool TB, int TILE, int KP>
 __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
                                           int lda, const glbf* __restrict__ B, int ldb, double beta,
                                           glbf* __restrict__ C, int ldc, ldsf* lds) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
-    constexpr int LDT = TILE + 4;
-    constexpr int E = TILE * 16 / NT;  // staged elements per thread and operand
+    constexpr int LDT = TILE + 1;
+    constexpr int E = TILE * KP / NT;  // staged elements per thread and operand
     constexpr int FM = TILE / 32, FN = TILE / 64;
-    ldsf* As = lds;
-    ldsf* Bs = lds + 16 * LDT;
+    constexpr int STAGE = 2 * KP * LDT;
+    // arguments of an out-of-line call arrive in vector registers; they are
+    // uniform, and saying so turns the tile and fragment tests into scalar
+    // branches instead of exec-masked code
+    M = __builtin_amdgcn_readfirstlane(M);
+    N = __builtin_amdgcn_readfirstlane(N);
+    K = __builtin_amdgcn_readfirstlane(K);
+    lda = __builtin_amdgcn_readfirstlane(lda);
+    ldb = __builtin_amdgcn_readfirstlane(ldb);
+    ldc = __builtin_amdgcn_readfirstlane(ldc);
     const int t = threadIdx.x;
-    const int w = t / WAVE, lane = t % WAVE, li = lane % 16, lg = lane / 16;
+    // wave index made scalar, so that per-wave decisions are scalar branches
+    const int w = __builtin_amdgcn_readfirstlane(t / WAVE);
+    const int lane = t % WAVE, li = lane % 16, lg = lane / 16;
     const int wm = w / 4, wn = w % 4;
     const int tm = (M + TILE - 1) / TILE, tn = (N + TILE - 1) / TILE;
+    // element e of this thread: (row along the contiguous direction)
+    auto a_mk = [&](int e, int& m, int& k) {
+        const int idx = t + e * NT;
+        if (TA) { m = idx % TILE; k = idx / TILE; }   // A[k][m]: contiguous m
+        else { k = idx % KP; m = idx / KP; }          // A[m][k]: contiguous k
+    };
+    auto b_nk = [&](int e, int& n, int& k) {
+        const int idx = t + e * NT;
+        if (TB) { k = idx % KP; n = idx / KP; }       // B[n][k]: contiguous k
+        else { n = idx % TILE; k = idx / TILE; }      // B[k][n]: contiguous n
+    };
     for (int tile = 0; tile < tm * tn; ++tile) {
         const int m0 = (tile / tn) * TILE, n0 = (tile % tn) * TILE;
         // 16x16 fragments of this wave that hold any output (the last tile
-        // row/column is partial): the others skip their MFMAs (wave-uniform)
+        // row/column is partial): the others skip their MFMAs
         const int rrem = M - (m0 + wm * (TILE / 2)), crem = N - (n0 + wn * (TILE / 4));
         const int nfm = rrem <= 0 ? 0 : (rrem >= FM * 16 ? FM : (rrem + 15) / 16);
         const int nfn = crem <= 0 ? 0 : (crem >= FN * 16 ? FN : (crem + 15) / 16);
@@ -138,65 +164,84 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) acc[i][j] = v4zero();
-        // operand panels k0..k0+15 go global -> registers -> LDS; the next
-        // panel's global loads are issued before the current panel's MFMAs
         double pa[E], pb[E];
         auto fetch = [&](int k0) {
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 int m, k;
-                if (TA) { k = t / (TILE / E); m = (t % (TILE / E)) * E + e; }
-                else { m = t / (16 / E); k = (t % (16 / E)) * E + e; }
+                a_mk(e, m, k);
                 const int gm = m0 + m, gk = k0 + k;
                 pa[e] = (gm < M && gk < K) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 int n, k;
-                if (TB) { n = t / (16 / E); k = (t % (16 / E)) * E + e; }
-                else { k = t / (TILE / E); n = (t % (TILE / E)) * E + e; }
+                b_nk(e, n, k);
                 const int gn = n0 + n, gk = k0 + k;
                 pb[e] = (gn < N && gk < K) ? (TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn]) : 0.0;
             }
         };
-        fetch(0);
-        for (int k0 = 0; k0 < K; k0 += 16) {
+        auto store = [&](ldsf* st) {
+            ldsf* As = st;
+            ldsf* Bs = st + KP * LDT;
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 int m, k;
-                if (TA) { k = t / (TILE / E); m = (t % (TILE / E)) * E + e; }
-                else { m = t / (16 / E); k = (t % (16 / E)) * E + e; }
+                a_mk(e, m, k);
                 As[k * LDT + m] = pa[e];
             }
 #pragma unroll
             for (int e = 0; e < E; ++e) {
                 int n, k;
-                if (TB) { n = t / (16 / E); k = (t % (16 / E)) * E + e; }
-                else { k = t / (TILE / E); n = (t % (TILE / E)) * E + e; }
+                b_nk(e, n, k);
                 Bs[k * LDT + n] = pb[e];
             }
-            __syncthreads();
-            if (k0 + 16 < K) fetch(k0 + 16);
+        };
+        // MFMAs of one panel when every fragment of the wave holds output
+        auto panel_full = [&](const ldsf* As, const ldsf* Bs) {
 #pragma unroll
-            for (int kk = 0; kk < 4; ++kk) {
+            for (int kk = 0; kk < KP / 4; ++kk) {
                 double a[FM], b[FN];
 #pragma unroll
                 for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDT + wm * (TILE / 2) + i * 16 + li];
 #pragma unroll
                 for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDT + wn * (TILE / 4) + j * 16 + li];
-                if (full) {
 #pragma unroll
-                    for (int i = 0; i < FM; ++i)
+                for (int i = 0; i < FM; ++i)
 #pragma unroll
-                        for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
-                } else {
-#pragma unroll
-                    for (int i = 0; i < FM; ++i)
-#pragma unroll
-                        for (int j = 0; j < FN; ++j)
-                            if (i < nfm && j < nfn) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
-                }
+                    for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+                __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // DS reads
+                __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);  // MFMAs
             }
+        };
+        // partial tiles: fragments without output skip their MFMAs
+        auto panel_part = [&](const ldsf* As, const ldsf* Bs) {
+#pragma unroll
+            for (int kk = 0; kk < KP / 4; ++kk) {
+                double a[FM], b[FN];
+#pragma unroll
+                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDT + wm * (TILE / 2) + i * 16 + li];
+#pragma unroll
+                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDT + wn * (TILE / 4) + j * 16 + li];
+#pragma unroll
+                for (int i = 0; i < FM; ++i)
+#pragma unroll
+                    for (int j = 0; j < FN; ++j)
+                        if (i < nfm && j < nfn) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+            }
+        };
+        const int np = (K + KP - 1) / KP;
+        if (np > 0) {
+            fetch(0);
+            store(lds);
+            __syncthreads();
+        }
+        for (int p = 0; p < np; ++p) {
+            ldsf* cur = lds + (p & 1) * STAGE;
+            if (p + 1 < np) fetch((p + 1) * KP);
+            if (full) panel_full(cur, cur + KP * LDT);
+            else panel_part(cur, cur + KP * LDT);
+            if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
             __syncthreads();
         }
 #pragma unroll
@@ -216,14 +261,19 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
     __syncthreads();
 }
 
-constexpr int NGEMM_MFMA_LDS = 2 * 16 * (128 + 4);
+// k-panel depth of the workgroup GEMM
+#ifndef XTB_GEMM_KP
+#define XTB_GEMM_KP 16
+#endif
+constexpr int GEMM_KP = XTB_GEMM_KP;
+constexpr int NGEMM_MFMA_LDS = 4 * GEMM_KP * (128 + 1);
 
 // dispatch: matrix-core GEMM with a tile matched to the size
 template <bool TA, bool TB>
 __device__ inline void gemm(int M, int N, int K, double alpha, const glbf* A, int lda,
                             const glbf* B, int ldb, double beta, glbf* C, int ldc, ldsf* lds) {
-    if (M <= 64 && N <= 64) wg_gemm_mfma<TA, TB, 64>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
-    else wg_gemm_mfma<TA, TB, 128>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
+    if (M <= 64 && N <= 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
+    else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
 }
 
 // ---------------------------------------------------------------------------
