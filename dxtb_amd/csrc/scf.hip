@@ -185,6 +185,121 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
 }
 
 // ---------------------------------------------------------------------------
+// Orthogonaliser from the Cholesky factor S = L L^T: the rows of X = L^-1 are
+// S-orthonormal (X S X^T = I), so Ct <- X starts the SCF like the Loewdin
+// X = S^-1/2 does (eigenpairs of the generalised problem do not depend on the
+// choice), at about n^3 flops instead of Newton-Schulz's ~10 x 6 n^3.
+// Blocked right-looking factorisation with 32-wide panels in T (lower
+// triangle): each diagonal block is factored and inverted by one wave in
+// LDS; the panel below (x D^T = b, i.e. x = b D^-T) and the trailing update
+// are matrix-core GEMMs.  X follows by block forward substitution L X = I:
+// X_i = D_i^-1 (E_i - L_i,<i X_<i), again with GEMMs.  A block padded beyond
+// n is the identity.  Scratch: F (up to 32 x n, or mt x 32 with stride n),
+// P (the nb x nb inverse, stride nb).
+// Returns 0; a non-positive pivot sets XTB_MOL_JACOBI in *status.
+// ---------------------------------------------------------------------------
+__device__ void chol_block_inverse(ldsf* D, ldsf* Di, glbf* Dg, int nb, int lane) {
+    // lower-triangular D (NB x NB, LDS) -> Di = D^-1 (LDS) and its leading
+    // nb x nb block to Dg (global, leading dimension nb: it must fit in an
+    // n x n buffer when n < NB); lane j forms column j by forward substitution
+    constexpr int NB = 32;
+    if (lane < NB) {
+        const int j = lane;
+        for (int i = 0; i < NB; ++i) {
+            double v;
+            if (i < j) v = 0.0;
+            else if (i == j) v = 1.0 / D[j * NB + j];
+            else {
+                double acc = 0.0;
+                for (int k = j; k < i; ++k) acc = fma(D[i * NB + k], Di[k * NB + j], acc);
+                v = -acc / D[i * NB + i];
+            }
+            Di[i * NB + j] = v;
+            if (i < nb && j < nb) Dg[i * nb + j] = v;
+        }
+    }
+}
+
+__device__ __noinline__ int chol_setup(const Mol& M, const Shared& sh, int* status) {
+    constexpr int NB = 32;
+    const int n = M.n;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    glbf* L = M.T;
+    glbf* X = M.Ct;
+    glbf* R = M.F;   // NB x n (and mt x NB) scratch
+    glbf* Dg = M.P;  // NB x NB inverse of the diagonal block
+    // LDS: two 32 x 32 blocks at the end of the GEMM staging area; every GEMM
+    // below runs after the blocks were consumed (the inverse goes through Dg)
+    ldsf* D = sh.big + NGEMM_MFMA_LDS - 2 * NB * NB;
+    ldsf* Di = D + NB * NB;
+    for (int r = w; r < n; r += NWAVE)
+        for (int c = lane; c < n; c += WAVE) L[(size_t)r * n + c] = M.S[(size_t)r * n + c];
+    __syncthreads();
+    bool bad = false;
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nb = n - j0 < NB ? n - j0 : NB;
+        for (int e = threadIdx.x; e < NB * NB; e += NT) {
+            const int r = e / NB, c = e % NB;
+            D[e] = (r < nb && c < nb) ? L[(size_t)(j0 + r) * n + j0 + c] : (r == c ? 1.0 : 0.0);
+        }
+        __syncthreads();
+        if (w == 0) {
+            for (int c = 0; c < NB; ++c) {
+                const double d = D[c * NB + c];
+                wave_sync();
+                const double sd = d > 0.0 ? sqrt(d) : 1.0;
+                if (!(d > 0.0)) bad = true;
+                if (lane == 0) D[c * NB + c] = sd;
+                for (int r = c + 1 + lane; r < NB; r += WAVE) D[r * NB + c] /= sd;
+                wave_sync();
+                const int m = NB - c - 1;
+                for (int e = lane; e < m * m; e += WAVE) {
+                    const int rr = c + 1 + e / m, qq = c + 1 + e % m;
+                    if (qq <= rr) D[rr * NB + qq] -= D[rr * NB + c] * D[qq * NB + c];
+                }
+                wave_sync();
+            }
+            chol_block_inverse(D, Di, Dg, nb, lane);
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < nb * nb; e += NT) {
+            const int r = e / nb, c = e % nb;
+            if (c <= r) L[(size_t)(j0 + r) * n + j0 + c] = D[r * NB + c];
+        }
+        const int mt = n - j0 - nb;
+        if (mt > 0) {
+            glbf* Pn = L + (size_t)(j0 + nb) * n + j0;  // panel below the block
+            gemm<false, true>(mt, nb, nb, 1.0, Pn, n, Dg, nb, 0.0, R, n, sh.big);  // b D^-T
+            for (int r = w; r < mt; r += NWAVE)
+                for (int c = lane; c < nb; c += WAVE) Pn[(size_t)r * n + c] = R[(size_t)r * n + c];
+            __syncthreads();
+            gemm<false, true>(mt, mt, nb, -1.0, Pn, n, Pn, n, 1.0, Pn + nb, n, sh.big);
+        } else {
+            __syncthreads();
+        }
+    }
+    // X = L^-1 by block rows: X_i = D_i^-1 (E_i - L_i,<i X_<i)
+    for (int r = w; r < n; r += NWAVE)
+        for (int c = lane; c < n; c += WAVE) X[(size_t)r * n + c] = 0.0;
+    __syncthreads();
+    for (int i0 = 0; i0 < n; i0 += NB) {
+        const int nb = n - i0 < NB ? n - i0 : NB;
+        if (i0 > 0) gemm<false, false>(nb, i0, i0, -1.0, L + (size_t)i0 * n, n, X, n, 0.0, R, n, sh.big);
+        for (int e = threadIdx.x; e < NB * NB; e += NT) {
+            const int r = e / NB, c = e % NB;
+            D[e] = (r < nb && c < nb) ? (c <= r ? L[(size_t)(i0 + r) * n + i0 + c] : 0.0) : (r == c ? 1.0 : 0.0);
+            if (r < nb && c < nb) R[(size_t)r * n + i0 + c] = (r == c) ? 1.0 : 0.0;
+        }
+        __syncthreads();
+        if (w == 0) chol_block_inverse(D, Di, Dg, nb, lane);
+        __syncthreads();
+        gemm<false, false>(nb, i0 + nb, nb, 1.0, Dg, nb, R, n, 0.0, X + (size_t)i0 * n, n, sh.big);
+    }
+    if (__syncthreads_or(bad)) *status |= XTB_MOL_JACOBI;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // Fermi smearing of one spin channel (src/dxtb/_src/wavefunction/
 // filling.py:201-366): mid-gap start, Newton on the electron count, exponent
 // cutoff 50, threshold on |N - sum f|.  Sorted eigenvalues in sh.skey.
@@ -764,7 +879,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     double tacc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     sh.tacc = tacc;
     const double tstart = now_us();
-    int sweeps = ortho_setup(M, sh, &status);
+    int sweeps = o.ortho == 1 ? chol_setup(M, sh, &status) : ortho_setup(M, sh, &status);
     tacc[0] = now_us() - tstart;
     double ef[2] = {0.0, 0.0};
 

@@ -199,6 +199,9 @@ class ScfOptions:
     pd_kmax: float = 0.05
     # eigensolve tolerance of the first SCF step (inexact SCF only)
     first_tol: float = 1e-2
+    # orthogonaliser of the first step: 0 Loewdin S^-1/2 (Newton-Schulz),
+    # 1 Cholesky L^-1 (same eigenpairs, ~10x fewer flops)
+    ortho: int = 1
 
     def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
@@ -210,7 +213,7 @@ class ScfOptions:
             x_atol=self.x_atol, x_atol_max=self.x_atol_max, kt=self.fermi_etemp * C.KELVIN2AU,
             damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
             fermi_thresh=thr, jacobi_tol=self.jacobi_tol, bd_gap_kt=self.bd_gap_kt,
-            pd_kmax=self.pd_kmax, first_tol=self.first_tol,
+            pd_kmax=self.pd_kmax, first_tol=self.first_tol, ortho=self.ortho,
         )
 
 

@@ -185,6 +185,8 @@ typedef struct xtb_scf_opts {
                                 amplitude is <= this (0: Jacobi only) */
     double first_tol;        /* eigensolve tolerance of the first SCF step
                                 (used when eig_inexact > 0) */
+    int32_t ortho;           /* orthogonaliser: 0 Loewdin S^-1/2, 1 Cholesky L^-1 */
+    int32_t pad1;
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */

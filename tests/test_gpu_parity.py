@@ -154,3 +154,17 @@ def test_block_diagonal_steps_reach_the_same_fixed_point(fx):
             ea, eb = a.emo[int(a.L.vec0[k]):int(a.L.vec0[k]) + int(a.L.n[k])], \
                 b.emo[int(b.L.vec0[k]):int(b.L.vec0[k]) + int(b.L.n[k])]
             assert (ea - eb).abs().max().item() < 1e-8
+
+
+@pytest.mark.parametrize("key", ["h2", "h2o", "ch4"])
+def test_small_molecule_alone_vs_oracle(fx, key):
+    """A molecule alone in its batch: every n x n work buffer is exactly
+    n^2 long (n = 2..12 < 32, the width of the Cholesky and Jacobi blocks),
+    so a block routine writing past n^2 would corrupt the other buffers."""
+    z, p = mol(fx, key)
+    print(f"case: {key} alone", flush=True)
+    pipe = run_native([(z, p)], forces=True)
+    r = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
+    assert abs(pipe.energies().item() - r["energy"]) < 1e-9
+    assert np.abs(pipe.gradient().cpu().numpy() - r["gradient"].numpy()).max() < 1e-8
+    assert np.abs(pipe.orbital_charges(0).cpu().numpy() - r["charges_orbital"].numpy()).max() < 1e-8
