@@ -168,3 +168,15 @@ def test_small_molecule_alone_vs_oracle(fx, key):
     assert abs(pipe.energies().item() - r["energy"]) < 1e-9
     assert np.abs(pipe.gradient().cpu().numpy() - r["gradient"].numpy()).max() < 1e-8
     assert np.abs(pipe.orbital_charges(0).cpu().numpy() - r["charges_orbital"].numpy()).max() < 1e-8
+
+
+def test_loewdin_and_cholesky_orthogonalisers_agree(fx):
+    """The first-step basis (Loewdin S^-1/2 or Cholesky L^-1) changes the SCF
+    path, not the converged eigenpairs: results agree to the parity bar."""
+    rng = np.random.default_rng(10)
+    mols = [mol(fx, "lys_xao"), mol(fx, "ch4"), W.chain(rng, 30), W.water_cluster(rng, 12)]
+    print("case: Loewdin vs Cholesky orthogonaliser", flush=True)
+    a = run_native(mols, forces=True, ortho=0)
+    b = run_native(mols, forces=True, ortho=1)
+    assert np.abs(a.energies().cpu().numpy() - b.energies().cpu().numpy()).max() < 1e-9
+    assert np.abs(a.gradient().cpu().numpy() - b.gradient().cpu().numpy()).max() < 1e-8
