@@ -422,7 +422,9 @@ __device__ inline void fock_build(const Mol& M, const glbf* vin) {
 // ask for the Jacobi eigensolve with Ct and F untouched, -1 to ask for it
 // after F was used as scratch (the caller rebuilds F).  Jacobi is asked for
 // when an amplitude exceeds pd_kmax, a denominator is not positive, k > n/2,
-// or a series would need more than 9 terms.
+// a series would need more than 9 terms, or (exact calls, loose = 0) two
+// passes leave a coupling above the tolerance.  On return 1, sh.ev holds the
+// diagonal of C F C^T in the rows' order (occupied rows: of the last pass).
 // Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the
 // Horner accumulators.
 // ---------------------------------------------------------------------------
@@ -545,7 +547,9 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         __syncthreads();
         if (kmax * aov <= tol) return 1;  // second-order remainder below the tolerance
     }
-    return 1;
+    // two passes: good enough for an inexact step; an exact one (loose = 0)
+    // asks for Jacobi on the rotated orbitals instead
+    return loose > 0.0 ? 1 : -1;
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +559,7 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
                         const xtb_params& par, const xtb_scf_opts& o, const glbf* vin,
                         glbf* vout, int* status, double* ef, double loose, bool allow_bd,
-                        bool sort_rows, bool final_step = false) {
+                        bool sort_rows, bool final_step = false, int* decoupled = nullptr) {
     const int n = M.n;
     // Occupation classes for this step's eigensolve.  With integer alpha/beta
     // counts and frontier gaps of at least bd_gap_kt x kT in the previous
@@ -583,13 +587,18 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     if (final_step) {
         // Final (exact) step of a gapped closed shell: remove the last
         // occupied-virtual coupling perturbatively (it is second order after
-        // that), then diagonalise inside the two classes only.
+        // that) to the full tolerance.  That fixes the density of the 0/1
+        // occupations, the energies and W (rotations inside the occupied
+        // block leave C^T f C and C_occ^T A_oo C_occ unchanged); only the
+        // orbital energies need the diagonalisation inside the two classes.
         if (cb_lo >= 0 && cb_lo == cb_hi && o.pd_kmax > 0.0) {
             const int r = pd_step(M, sh, o, cb_lo, 0.0, vin);
             intra = r > 0;
             if (r != 0) fock_build(M, vin);  // F may have served as scratch
         }
         if (!intra) cb_lo = cb_hi = -1;
+        pd = intra && !o.orbitals;
+        if (pd && decoupled) *decoupled = cb_lo;
     } else if (cb_lo >= 0 && cb_lo == cb_hi && o.pd_kmax > 0.0) {
         const int r = pd_step(M, sh, o, cb_lo, loose, vin);
         pd = r > 0;
@@ -915,8 +924,9 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     if (o.maxiter == 0) converged = true;
     // converged_to_charges (scf/base.py:472-513): density of the last SCF
     // output potential; the potential of its charges goes to st.pot
+    int kdec = 0;  // > 0: the final step stopped at the decoupling, k occupied rows first
     const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false, false,
-                                 o.eig_inexact > 0.0);
+                                 o.eig_inexact > 0.0, &kdec);
     sweeps += sw_last;
 
     // energies (scf/base.py:514-623)
@@ -940,8 +950,26 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
             }
     }
     wg_sum_n<4>(acc, sh.red);
-    for (int j = threadIdx.x; j < M.n; j += NT) M.emo[j] = sh.skey[j];
-    if (o.want_w) {
+    for (int j = threadIdx.x; j < M.n; j += NT) M.emo[j] = kdec > 0 ? NAN : sh.skey[j];
+    if (o.want_w && kdec > 0) {
+        // W = C diag(f e) C^T (analytical.py:186-190) from the occupied block
+        // A = C_occ F C_occ^T without diagonalising it: with the occupations
+        // f_i of the rows, W = C_occ^T A' C_occ, A'_ij = (f_i + f_j)/2 A_ij,
+        // equal to the eigenbasis sum for uniform f (here 2 to ~1e-13; the
+        // virtual rows' Fermi tails are below that).  F of the final step's
+        // input potential is rebuilt; A and A' C_occ live in F (k <= n/2).
+        const int n = M.n, k = kdec;
+        const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        fock_build(M, M.vnew);
+        gemm<false, false>(k, n, n, 1.0, M.Ct, n, M.F, n, 0.0, M.T, n, sh.big);
+        gemm<false, true>(k, k, n, 1.0, M.T, n, M.Ct, n, 0.0, M.F, k, sh.big);
+        for (int i = w; i < k; i += NWAVE)
+            for (int j = lane; j < k; j += WAVE) M.F[(size_t)i * k + j] *= 0.5 * (sh.ftot[i] + sh.ftot[j]);
+        __syncthreads();
+        glbf* R = M.F + (size_t)k * k;
+        gemm<false, false>(k, n, k, 1.0, M.F, k, M.Ct, n, 0.0, R, n, sh.big);
+        gemm<true, false>(n, n, k, 1.0, M.Ct, n, R, n, 0.0, M.T, n, sh.big);
+    } else if (o.want_w) {
         // W = C diag(f e) C^T (analytical.py:186-190) over the occupied
         // orbitals; the orbitals are not needed afterwards, so Ct takes f e C
         const int kocc = gather_occupied(M, sh, M.F, M.Ct, true);

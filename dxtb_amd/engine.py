@@ -202,6 +202,10 @@ class ScfOptions:
     # orthogonaliser of the first step: 0 Loewdin S^-1/2 (Newton-Schulz),
     # 1 Cholesky L^-1 (same eigenpairs, ~10x fewer flops)
     ortho: int = 1
+    # form the orbital energies in the final step (a full eigensolve); off,
+    # gapped closed shells stop at the occupied-virtual decoupling, which
+    # fixes the density, energies and W but not the orbital energies
+    orbital_energies: bool = False
 
     def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
@@ -214,6 +218,7 @@ class ScfOptions:
             damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
             fermi_thresh=thr, jacobi_tol=self.jacobi_tol, bd_gap_kt=self.bd_gap_kt,
             pd_kmax=self.pd_kmax, first_tol=self.first_tol, ortho=self.ortho,
+            orbitals=int(self.orbital_energies),
         )
 
 

@@ -82,7 +82,7 @@ extern "C" {
 #define XTB_OUT_EREP 9      /* written by xtb_repulsion */
 #define XTB_OUT_SWEEPS 10   /* total Jacobi sweeps spent */
 /* stage timers of the SCF kernel, microseconds (device wall clock) */
-#define XTB_OUT_T_ORTHO 11  /* orthogonaliser S^-1/2 (Newton-Schulz) */
+#define XTB_OUT_T_ORTHO 11  /* orthogonaliser (Cholesky L^-1 or Loewdin S^-1/2) */
 #define XTB_OUT_T_GEMM 12   /* Fock transform, density and W products */
 #define XTB_OUT_T_EIGEN 13  /* eigensolves / decoupling of the SCF steps */
 #define XTB_OUT_T_TOTAL 14  /* whole molecule */
@@ -186,7 +186,12 @@ typedef struct xtb_scf_opts {
     double first_tol;        /* eigensolve tolerance of the first SCF step
                                 (used when eig_inexact > 0) */
     int32_t ortho;           /* orthogonaliser: 0 Loewdin S^-1/2, 1 Cholesky L^-1 */
-    int32_t pad1;
+    int32_t orbitals;        /* 1: the final step forms the orbital energies (emo)
+                                by a full eigensolve; 0: for gapped closed shells
+                                it stops once occupied and virtual orbitals are
+                                decoupled (density, energies and W are the same),
+                                emo is then NaN and the Fermi levels are mid-gap
+                                values of the decoupled blocks' diagonals */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */
