@@ -2,8 +2,13 @@
 
 Workload: the serving stream of log-uniform 5..200-atom molecules (70 % CHNO
 chains, 30 % water clusters), energy + analytic forces in FP64, seed 4.  A
-"step" serves the next `--chunk` requests of the stream with all of them
-available at the start (saturation throughput).  The JSON line also carries
+"step" hands the next `--chunk` requests of the stream, all available at
+once, to the continuous-batching serving loop, which admits them as streams
+free up while the previous chunk's molecules finish (saturation throughput
+of the loop; the timed region ends when every chunk is done).  The JSON line
+also carries
+
+* the same chunks served one at a time, each drained before the next starts;
 
 * the plain padded-batch drop-in on the same requests (arrival order,
   `--padded-batch` molecules per padded batch) and the ragged/padded ratio;
@@ -162,6 +167,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-padded", action="store_true")
+    ap.add_argument("--no-barrier", action="store_true",
+                    help="skip the comparison run that drains each chunk before the next")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--profile-run", action="store_true",
                     help="serve one chunk only (for rocprofv3 runs), print nothing else")
@@ -215,20 +222,39 @@ def main() -> int:
         server.run_stream(_requests(stream, pos, a.chunk), realtime=False)
         pos += a.chunk
         log(f"warmup {w} done")
+    # continuous batching: step k hands chunk k to the loop, which admits it
+    # as streams free up while chunk k-1 finishes; the step ends when chunk
+    # k-1 is done, and the timed region ends when every chunk is done
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    bad = 0
+    served, prev = [], []
     for k in range(a.steps):
         reqs = _requests(stream, pos, a.chunk)
-        server.run_stream(reqs, realtime=False)
-        bad += sum(1 for r in reqs if server.results[r.rid].status != 0)
+        server.feed(reqs)
+        server.serve_until(prev)
+        served += reqs
+        prev = reqs
         pos += a.chunk
-        log(f"step {k} done")
+        log(f"step {k} handed over")
+    server.serve_until(prev)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    bad = sum(1 for r in served if server.results[r.rid].status != 0)
     value = a.steps * a.chunk / (t1 - t0)
     ms_step = (t1 - t0) * 1e3 / a.steps
     log(f"ragged: {value:.1f} molecules/s")
+    # the same chunks one at a time, each drained before the next starts
+    barrier = None
+    if not a.no_barrier:
+        bsrv = RaggedServer(dev, opts=opts)
+        bsrv.run_stream(_requests(stream, 0, a.chunk), realtime=False)  # warm-up
+        torch.cuda.synchronize()
+        tb0 = time.perf_counter()
+        for k in range(a.steps):
+            bsrv.run_stream(_requests(stream, (a.warmup + k) * a.chunk, a.chunk), realtime=False)
+        torch.cuda.synchronize()
+        barrier = a.steps * a.chunk / (time.perf_counter() - tb0)
+        log(f"ragged, chunks drained one at a time: {barrier:.1f} molecules/s")
 
     # roofline of the SCF kernel: one stream, launches do not overlap
     rp = roofline_pass(stream, a.warmup * a.chunk, a.chunk, dev, opts)
@@ -272,6 +298,9 @@ def main() -> int:
             "workload": "serving stream (log-uniform 5-200 atoms; 70% CHNO chains, 30% water "
                         "clusters), energy + analytic forces, FP64, SCF x_atol 1e-10",
             "requests_per_step": a.chunk, "seed": 4,
+            "batching": "continuous: step k hands its chunk to the serving loop, which admits "
+                        "it while chunk k-1 finishes",
+            "chunk_barrier_molecules_per_s": None if barrier is None else round(barrier, 3),
             "padded_dropin_molecules_per_s": None if padded is None else round(padded, 3),
             "ragged_over_padded": None if padded is None else round(value / padded, 3),
             "padded_batch": a.padded_batch,

@@ -212,6 +212,23 @@ class RaggedServer:
             pass
         return done
 
+    def feed(self, reqs: list[Request], now: float | None = None) -> None:
+        """Submit requests that arrive together now and admit what fits."""
+        now = time.perf_counter() if now is None else now
+        for r in reqs:
+            r.arrival = now
+            self.submit(r)
+        self.step(now)
+
+    def serve_until(self, reqs: list[Request]) -> None:
+        """Run the loop (admitting anything waiting, in arrival order) until
+        every request of `reqs` has its result."""
+        want = [r.rid for r in reqs]
+        while any(rid not in self.results for rid in want):
+            self.step()
+            if not self.free_streams or not self.pending():
+                self.poll(block=True)
+
     def drain(self) -> None:
         while self.pending() or self.inflight:
             self.step()
