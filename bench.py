@@ -159,7 +159,11 @@ def measured_traffic(chunk: int, warmup: int) -> float | None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--lookahead", type=int, default=4,
+                    help="chunks the serving loop holds (waiting or in flight) at most")
+    ap.add_argument("--hw-queues", type=int, default=8,
+                    help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-32)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=600)
     ap.add_argument("--latency-requests", type=int, default=600)
@@ -182,6 +186,14 @@ def main() -> int:
               "(molecules are independent: shard requests across GPUs by a router).",
               file=sys.stderr)
         return 2
+
+    # more hardware queues let more micro-batches run side by side (one
+    # stream per queue); it must be set before HIP initialises
+    if not 1 <= a.hw_queues <= 32:
+        print("--hw-queues must be in 1..32", file=sys.stderr)
+        return 2
+    os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
+    import gc
 
     import torch
 
@@ -222,21 +234,27 @@ def main() -> int:
         server.run_stream(_requests(stream, pos, a.chunk), realtime=False)
         pos += a.chunk
         log(f"warmup {w} done")
+    # the start-up heap (the request stream, torch) is frozen so that Python's
+    # full collections, ~0.1 s each over it, do not stall the host loop
+    gc.collect()
+    gc.freeze()
     # continuous batching: step k hands chunk k to the loop, which admits it
-    # as streams free up while chunk k-1 finishes; the step ends when chunk
-    # k-1 is done, and the timed region ends when every chunk is done
+    # as streams free up while earlier chunks finish; once `lookahead` chunks
+    # are held, the step ends when the oldest of them is done.  The timed
+    # region ends when every chunk is done.
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    served, prev = [], []
+    served, held = [], []
     for k in range(a.steps):
         reqs = _requests(stream, pos, a.chunk)
         server.feed(reqs)
-        server.serve_until(prev)
+        held.append(reqs)
+        if len(held) >= a.lookahead:
+            server.serve_until(held.pop(0))
         served += reqs
-        prev = reqs
         pos += a.chunk
         log(f"step {k} handed over")
-    server.serve_until(prev)
+    server.serve_until(served)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     bad = sum(1 for r in served if server.results[r.rid].status != 0)
@@ -298,8 +316,9 @@ def main() -> int:
             "workload": "serving stream (log-uniform 5-200 atoms; 70% CHNO chains, 30% water "
                         "clusters), energy + analytic forces, FP64, SCF x_atol 1e-10",
             "requests_per_step": a.chunk, "seed": 4,
-            "batching": "continuous: step k hands its chunk to the serving loop, which admits "
-                        "it while chunk k-1 finishes",
+            "batching": f"continuous: step k hands its chunk to the serving loop, which holds "
+                        f"at most {a.lookahead} chunks and admits each as streams free up",
+            "streams": server.streams.__len__(), "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
             "chunk_barrier_molecules_per_s": None if barrier is None else round(barrier, 3),
             "padded_dropin_molecules_per_s": None if padded is None else round(padded, 3),
             "ragged_over_padded": None if padded is None else round(value / padded, 3),

@@ -26,6 +26,8 @@ MOL_JACOBI = 8
 OUT_STRIDE = 32
 OUT_ESCF, OUT_EH0, OUT_EES2, OUT_EES3, OUT_FENERGY = 0, 1, 2, 3, 4
 OUT_ITER, OUT_STATUS, OUT_EF_A, OUT_EF_B, OUT_EREP, OUT_SWEEPS = 5, 6, 7, 8, 9, 10
+OUT_T_TOTAL = 14  # SCF kernel device time of the molecule, microseconds
+OUT_T_START = 20  # device wall clock at the molecule's start, microseconds
 
 P = ct.c_void_p
 i32 = ct.c_int32

@@ -992,6 +992,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_T_GEMM] = tacc[1];
         out[XTB_OUT_T_EIGEN] = tacc[2];
         out[XTB_OUT_T_TOTAL] = now_us() - tstart;
+        out[XTB_OUT_T_START] = tstart;
         out[XTB_OUT_T_BSUB] = tacc[4];
         out[XTB_OUT_T_BROW] = tacc[5];
         out[XTB_OUT_PD_STEPS] = tacc[6];

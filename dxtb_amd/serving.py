@@ -12,9 +12,9 @@ Requests (one molecule each, arbitrary size) arrive over time.  The server
   analytic forces) for the micro-batch on its own HIP stream, so launches
   overlap on the device: one workgroup per molecule, and workgroups of a
   newer launch fill compute units that molecules of an older launch have
-  released by converging.  Three streams by default: HIP maps streams onto
-  the process's hardware queues (four, one of them taken by the default
-  stream), and two streams on one queue run their work one after the other;
+  released by converging.  One stream per hardware queue but the default
+  stream's (GPU_MAX_HW_QUEUES - 1, three with HIP's default of four): two
+  streams on one queue run their work one after the other;
 * keeps each molecule's SCF state (potential, Anderson history, orbitals,
   density) resident in that launch's device buffers until its results are
   copied out, then frees the buffers and backfills with waiting requests.
@@ -25,6 +25,8 @@ neighbours are; ``tests/test_gpu_serving.py`` checks alone == in the mix.
 
 from __future__ import annotations
 
+import functools
+import os
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -32,6 +34,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
+from . import _lib
 from .engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
 from .param import SpeciesTables, species_shells
 
@@ -70,14 +73,40 @@ class ServerStats:
     admission_stalls: int = 0
     peak_bytes: int = 0
     latencies: list = field(default_factory=list)
+    scf_busy_s: float = 0.0    # sum over molecules of the SCF kernel's per-molecule device time
+    host_admit_s: float = 0.0  # host time spent building and launching micro-batches
+    host_retire_s: float = 0.0  # host time spent copying results out
+    spans: list = field(default_factory=list)  # (start_us, duration_us) per molecule, when timing
+    launch_log: list = field(default_factory=list)  # per launch, when timing (see _retire)
+
+
+def default_streams() -> int:
+    """One stream per hardware queue of the process but the default
+    stream's: GPU_MAX_HW_QUEUES (HIP's default 4) minus one."""
+    return max(1, int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) - 1)
+
+
+@functools.lru_cache(maxsize=None)
+def _orbital_table() -> np.ndarray:
+    """Orbitals per element, indexed by atomic number (-1: unsupported)."""
+    from . import constants as C
+
+    tab = np.full(128, -1, dtype=np.int64)
+    for z in C.SUPPORTED_ELEMENTS:
+        tab[z] = sum(2 * l + 1 for l, *_ in species_shells(int(z)))
+    return tab
 
 
 def orbital_count(numbers: np.ndarray) -> int:
-    return int(sum(sum(2 * l + 1 for l, *_ in species_shells(int(z))) for z in numbers))
+    per = _orbital_table()[np.asarray(numbers, dtype=np.int64)]
+    if (per < 0).any():
+        bad = int(np.asarray(numbers)[per < 0][0])
+        species_shells(bad)  # raises the unsupported-element error
+    return int(per.sum())
 
 
 class RaggedServer:
-    def __init__(self, device="cuda:0", opts: ScfOptions | None = None, streams: int = 3,
+    def __init__(self, device="cuda:0", opts: ScfOptions | None = None, streams: int | None = None,
                  edges=DEFAULT_EDGES, max_batch=DEFAULT_MAXB, hbm_budget_gb: float = 64.0,
                  elements=(1, 6, 7, 8), forces: bool = True, max_wait_s: float = 0.002,
                  timing: bool = False):
@@ -85,6 +114,8 @@ class RaggedServer:
         self.opts = opts or ScfOptions()
         self.tables = SpeciesTables(elements)
         self.dtables = DeviceTables(self.tables, self.device)
+        if streams is None:
+            streams = default_streams()
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
         self.free_streams = deque(self.streams)
         self.edges = tuple(edges)
@@ -115,14 +146,14 @@ class RaggedServer:
         return sum(f.bytes for f in self.inflight)
 
     # --------------------------------------------------------------- admission
-    def _pick_class(self, now: float) -> int | None:
+    def _pick_class(self, now: float, eager: bool = False) -> int | None:
         best, best_t = None, None
         for k, q in enumerate(self.queues):
             if not q:
                 continue
             full = len(q) >= self.max_batch[k]
             waited = now - q[0].arrival >= self.max_wait
-            if not (full or waited or not self.inflight):
+            if not (eager or full or waited or not self.inflight):
                 continue
             # oldest request first; among requests that arrived at the same
             # instant (a submitted batch) the larger class first, so that the
@@ -131,10 +162,10 @@ class RaggedServer:
                 best, best_t = k, q[0].arrival
         return best
 
-    def _admit(self, now: float) -> bool:
+    def _admit(self, now: float, eager: bool = False) -> bool:
         if not self.free_streams:
             return False
-        k = self._pick_class(now)
+        k = self._pick_class(now, eager)
         if k is None:
             return False
         q = self.queues[k]
@@ -148,6 +179,7 @@ class RaggedServer:
             self.stats.admission_stalls += 1
             return False
         stream = self.free_streams.popleft()
+        th = time.perf_counter()
         with torch.cuda.stream(stream):
             pipe = Pipeline(lay, self.dtables, self.device)
             pos = torch.from_numpy(np.concatenate([r.positions for r in take]).astype(np.float64))
@@ -155,7 +187,8 @@ class RaggedServer:
             pipe.run(pos, self.opts, forces=self.forces, stream=stream, timing=self.timing)
             ev = torch.cuda.Event()
             ev.record(stream)
-        self.inflight.append(_Flight(pipe, take, stream, ev, nbytes, now))
+        self.stats.host_admit_s += time.perf_counter() - th
+        self.inflight.append(_Flight(pipe, take, stream, ev, nbytes, th))
         self.stats.launches += 1
         self.stats.peak_bytes = max(self.stats.peak_bytes, self._inflight_bytes())
         return True
@@ -163,6 +196,7 @@ class RaggedServer:
     # -------------------------------------------------------------- retirement
     def _retire(self, f: _Flight) -> list[Request]:
         f.event.synchronize()
+        th = time.perf_counter()
         pipe = f.pipe
         if self.timing:
             ev0, ev1 = pipe.scf_events
@@ -182,6 +216,14 @@ class RaggedServer:
             self.results[r.rid] = r
             self.stats.latencies.append(now - r.arrival)
         self.stats.completed += len(f.reqs)
+        self.stats.scf_busy_s += float(rec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
+        if self.timing:
+            self.stats.spans += list(zip(rec[:, _lib.OUT_T_START], rec[:, _lib.OUT_T_TOTAL]))
+            # host admit time, retire time, molecules, max n, device first start, last end (us)
+            self.stats.launch_log.append((f.t_launch, th, len(f.reqs), int(pipe.L.n.max()),
+                                          float(rec[:, _lib.OUT_T_START].min()),
+                                          float((rec[:, _lib.OUT_T_START] + rec[:, _lib.OUT_T_TOTAL]).max())))
+        self.stats.host_retire_s += time.perf_counter() - th
         self.free_streams.append(f.stream)
         return f.reqs
 
@@ -205,10 +247,14 @@ class RaggedServer:
                         break
         return out
 
-    def step(self, now: float | None = None) -> list[Request]:
+    def step(self, now: float | None = None, eager: bool = False) -> list[Request]:
+        """Retire finished launches and admit micro-batches while streams
+        and memory allow.  A class waits up to max_wait for more requests
+        unless it is full, nothing is in flight, or `eager` (a batch that
+        arrived whole has nothing more to wait for)."""
         now = time.perf_counter() if now is None else now
         done = self.poll()
-        while self._admit(now):
+        while self._admit(now, eager):
             pass
         return done
 
@@ -218,7 +264,7 @@ class RaggedServer:
         for r in reqs:
             r.arrival = now
             self.submit(r)
-        self.step(now)
+        self.step(now, eager=True)
 
     def serve_until(self, reqs: list[Request]) -> None:
         """Run the loop (admitting anything waiting, in arrival order) until

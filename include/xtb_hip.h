@@ -91,6 +91,7 @@ extern "C" {
 #define XTB_OUT_PD_STEPS 17 /* SCF steps decoupled by the perturbative rotation */
 #define XTB_OUT_SWEEPS_FIRST 18  /* Jacobi sweeps of the first SCF step */
 #define XTB_OUT_SWEEPS_LAST 19   /* Jacobi sweeps of the final (exact) step */
+#define XTB_OUT_T_START 20  /* device wall clock at the molecule's start, microseconds */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {
