@@ -62,11 +62,33 @@ def cpu_baseline(stream, seconds: float) -> dict:
         if time.perf_counter() - t0 > seconds:
             break
     dt = time.perf_counter() - t0
-    return {"value": n / dt, "unit": "molecules/s", "cores": torch.get_num_threads(),
-            "kind": "port",
+    threads = torch.get_num_threads()
+    # the same requests on one thread, for a bounded sample
+    torch.set_num_threads(1)
+    t1 = time.perf_counter()
+    n1 = 0
+    for _, z, p in stream[:n]:
+        og.singlepoint(z, p, opts=opts, with_forces=True)
+        n1 += 1
+        if time.perf_counter() - t1 > seconds / 3:
+            break
+    dt1 = time.perf_counter() - t1
+    torch.set_num_threads(threads)
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": n / dt, "unit": "molecules/s", "cores": threads,
+            "kind": "port", "single_thread_value": n1 / dt1,
             "sample": f"first {n} requests of the same stream ({atoms} atoms), energy+forces, "
                       f"sequential, {dt:.1f} s, torch {torch.__version__} CPU FP64, "
-                      f"{os.cpu_count()} logical CPUs"}
+                      f"{threads} torch threads on {os.cpu_count()} logical CPUs ({model}); "
+                      f"single thread: first {n1} requests, {dt1:.1f} s"}
 
 
 def forces_cost(reqs, device, opts, repeats: int = 3) -> dict:
@@ -137,8 +159,13 @@ def roofline_pass(stream, start: int, count: int, device, opts) -> dict:
     ms = np.array([x[0] for x in rs.kernel_log])
     fl = np.array([x[1] for x in rs.kernel_log])
     achieved = float(fl.mean() / (ms.mean() * 1e-3) / 1e12)
+    # the same flops over the workgroups' own device time (one workgroup per
+    # molecule, one CU each): the kernel's efficiency while resident,
+    # independent of how many CUs a launch occupies
+    per_cu_peak = FP64_PEAK_TFLOPS * 1e12 / 256.0
+    resident = float(fl.sum() / (rs.stats.scf_busy_s * per_cu_peak))
     return {"achieved": achieved, "launches": int(len(ms)), "mean_launch_ms": float(ms.mean()),
-            "flops_per_launch": float(fl.mean())}
+            "flops_per_launch": float(fl.mean()), "resident_frac": resident}
 
 
 def measured_traffic(chunk: int, warmup: int) -> float | None:
@@ -332,6 +359,7 @@ def main() -> int:
                      "traffic": measured_traffic(a.chunk, a.warmup), "kernel": "xtb::k_scf",
                      "launches": rp["launches"], "mean_launch_ms": round(rp["mean_launch_ms"], 3),
                      "flops_per_launch": rp["flops_per_launch"],
+                     "resident_frac": round(rp["resident_frac"], 4),
                      "measured_peaks": peaks},
         "cpu_baseline": cpu,
     }

@@ -58,6 +58,9 @@ def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
         ("damp_diagonal_offset", "diag_offset"), ("x_atol", "x_atol"),
         ("x_atol_max", "x_atol_max"), ("fermi_etemp", "fermi_etemp"),
         ("fermi_maxiter", "fermi_maxiter"), ("fermi_thresh", "fermi_thresh"),
+        # not an upstream option: successive calls on the same molecules
+        # start from the previous call's orbitals and potential
+        ("warm_start", "warm_start"),
     ):
         if key in rest:
             setattr(o, attr, rest.pop(key))

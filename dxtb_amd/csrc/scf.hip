@@ -220,21 +220,20 @@ __device__ void chol_block_inverse(ldsf* D, ldsf* Di, glbf* Dg, int nb, int lane
     }
 }
 
-__device__ __noinline__ int chol_setup(const Mol& M, const Shared& sh, int* status) {
+// In-place blocked right-looking Cholesky L L^T = A of the n x n symmetric
+// positive definite A (lower triangle read and overwritten by L; the strict
+// upper triangle is left as it was), 32-wide panels: one wave factors and
+// inverts each diagonal block in LDS, the panel solve and the trailing
+// update are matrix-core GEMMs.  R (>= n rows x 32, leading dimension n) and
+// Dg (32 x 32) are global scratch.  Returns false when a pivot is not
+// positive.
+__device__ __noinline__ bool chol_factor(glbf* L, int n, glbf* R, glbf* Dg, const Shared& sh) {
     constexpr int NB = 32;
-    const int n = M.n;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    glbf* L = M.T;
-    glbf* X = M.Ct;
-    glbf* R = M.F;   // NB x n (and mt x NB) scratch
-    glbf* Dg = M.P;  // NB x NB inverse of the diagonal block
     // LDS: two 32 x 32 blocks at the end of the GEMM staging area; every GEMM
     // below runs after the blocks were consumed (the inverse goes through Dg)
     ldsf* D = sh.big + NGEMM_MFMA_LDS - 2 * NB * NB;
     ldsf* Di = D + NB * NB;
-    for (int r = w; r < n; r += NWAVE)
-        for (int c = lane; c < n; c += WAVE) L[(size_t)r * n + c] = M.S[(size_t)r * n + c];
-    __syncthreads();
     bool bad = false;
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = n - j0 < NB ? n - j0 : NB;
@@ -278,25 +277,83 @@ __device__ __noinline__ int chol_setup(const Mol& M, const Shared& sh, int* stat
             __syncthreads();
         }
     }
-    // X = L^-1 by block rows: X_i = D_i^-1 (E_i - L_i,<i X_<i)
+    return !__syncthreads_or(bad);
+}
+
+// Load the lower-triangular diagonal block i0 of L (nb x nb, NB-padded with
+// the identity) and its inverse into Dg (leading dimension nb).
+__device__ inline void chol_diag_inverse(const glbf* L, int n, int i0, int nb, glbf* Dg, const Shared& sh) {
+    constexpr int NB = 32;
+    ldsf* D = sh.big + NGEMM_MFMA_LDS - 2 * NB * NB;
+    ldsf* Di = D + NB * NB;
+    for (int e = threadIdx.x; e < NB * NB; e += NT) {
+        const int r = e / NB, c = e % NB;
+        D[e] = (r < nb && c < nb) ? (c <= r ? L[(size_t)(i0 + r) * n + i0 + c] : 0.0) : (r == c ? 1.0 : 0.0);
+    }
+    __syncthreads();
+    if (threadIdx.x < WAVE) chol_block_inverse(D, Di, Dg, nb, threadIdx.x);
+    __syncthreads();
+}
+
+// Cholesky orthogonaliser: L L^T = S, Ct <- X = L^-1 (rows S-orthonormal).
+// X by block rows, X_i = D_i^-1 (E_i - L_i,<i X_<i), where only the first
+// i0 + nb columns of block row i are non-zero.  L in T, R in F, Dg in P.
+__device__ __noinline__ int chol_setup(const Mol& M, const Shared& sh, int* status) {
+    constexpr int NB = 32;
+    const int n = M.n;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    glbf* L = M.T;
+    glbf* X = M.Ct;
+    glbf* R = M.F;   // NB x n (and mt x NB) scratch
+    glbf* Dg = M.P;  // NB x NB inverse of the diagonal block
+    for (int r = w; r < n; r += NWAVE)
+        for (int c = lane; c < n; c += WAVE) L[(size_t)r * n + c] = M.S[(size_t)r * n + c];
+    __syncthreads();
+    if (!chol_factor(L, n, R, Dg, sh)) *status |= XTB_MOL_JACOBI;
     for (int r = w; r < n; r += NWAVE)
         for (int c = lane; c < n; c += WAVE) X[(size_t)r * n + c] = 0.0;
     __syncthreads();
     for (int i0 = 0; i0 < n; i0 += NB) {
         const int nb = n - i0 < NB ? n - i0 : NB;
         if (i0 > 0) gemm<false, false>(nb, i0, i0, -1.0, L + (size_t)i0 * n, n, X, n, 0.0, R, n, sh.big);
-        for (int e = threadIdx.x; e < NB * NB; e += NT) {
-            const int r = e / NB, c = e % NB;
-            D[e] = (r < nb && c < nb) ? (c <= r ? L[(size_t)(i0 + r) * n + i0 + c] : 0.0) : (r == c ? 1.0 : 0.0);
-            if (r < nb && c < nb) R[(size_t)r * n + i0 + c] = (r == c) ? 1.0 : 0.0;
+        for (int e = threadIdx.x; e < nb * nb; e += NT) {
+            const int r = e / nb, c = e % nb;
+            R[(size_t)r * n + i0 + c] = (r == c) ? 1.0 : 0.0;
         }
-        __syncthreads();
-        if (w == 0) chol_block_inverse(D, Di, Dg, nb, lane);
-        __syncthreads();
+        chol_diag_inverse(L, n, i0, nb, Dg, sh);
         gemm<false, false>(nb, i0 + nb, nb, 1.0, Dg, nb, R, n, 0.0, X + (size_t)i0 * n, n, sh.big);
     }
-    if (__syncthreads_or(bad)) *status |= XTB_MOL_JACOBI;
     return 0;
+}
+
+// Warm start: the kept orbitals (rows of Ct, S_old-orthonormal, ascending
+// energy) are re-orthonormalised against the new overlap by Cholesky
+// Gram-Schmidt in their order, C <- L^-1 C with G = C S C^T = L L^T, so the
+// lowest orbitals stay within the span of the lowest kept ones.  G and L in
+// P, C S and Dg in T, R in F; the forward substitution C_i <- D_i^-1 (C_i -
+// L_i,<i C_<i) runs in place by block rows.  Returns false (Ct unusable,
+// start cold) when G is not numerically positive definite.
+__device__ __noinline__ bool warm_orthonormalise(const Mol& M, const Shared& sh) {
+    constexpr int NB = 32;
+    const int n = M.n;
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+    glbf* G = M.P;
+    glbf* R = M.F;
+    glbf* Dg = M.T;
+    gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.S, n, 0.0, M.T, n, sh.big);  // C S
+    gemm<false, true>(n, n, n, 1.0, M.T, n, M.Ct, n, 0.0, G, n, sh.big);     // C S C^T
+    if (!chol_factor(G, n, R, Dg, sh)) return false;
+    for (int i0 = 0; i0 < n; i0 += NB) {
+        const int nb = n - i0 < NB ? n - i0 : NB;
+        glbf* Ci = M.Ct + (size_t)i0 * n;
+        for (int r = w; r < nb; r += NWAVE)
+            for (int c = lane; c < n; c += WAVE) R[(size_t)r * n + c] = Ci[(size_t)r * n + c];
+        __syncthreads();
+        if (i0 > 0) gemm<false, false>(nb, n, i0, -1.0, G + (size_t)i0 * n, n, M.Ct, n, 1.0, R, n, sh.big);
+        chol_diag_inverse(G, n, i0, nb, Dg, sh);
+        gemm<false, false>(nb, n, nb, 1.0, Dg, nb, R, n, 0.0, Ci, n, sh.big);
+    }
+    return true;
 }
 
 // ---------------------------------------------------------------------------
@@ -605,7 +662,9 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         if (r < 0) fock_build(M, vin);  // F served as scratch
     }
     if (pd) {
-        sh.tacc[2] += now_us() - t0;
+        const double dt = now_us() - t0;
+        sh.tacc[2] += dt;
+        sh.tacc[3] += dt;
         sh.tacc[6] += 1.0;
     } else {
         // T = F C ; A = C^T T  -> F
@@ -714,6 +773,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
 // ---------------------------------------------------------------------------
 struct Mixer {
     int step, head, G;
+    bool soft;  // upstream's soft start; a warm start, already near the fixed point, skips it
 };
 
 __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh, const xtb_scf_opts& o,
@@ -744,7 +804,7 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
     *resid_inf = linf;
     double theta[5] = {0, 0, 0, 0, 0};
     int nn = 0;
-    if (mx.step > gen) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
+    if (mx.step > gen || !mx.soft) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
     if (nn > 0) {
         // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn (nn <= 5)
         double acc[20];
@@ -820,7 +880,7 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
             }
             xm = xb + o.damp * fb;
         } else {
-            xm = x0[i] + o.damp_init * f0[i];
+            xm = x0[i] + (mx.soft ? o.damp_init : o.damp) * f0[i];
         }
         xout[i] = xm;
         xn[i] = xm;  // slot newhead may alias X(G-1): element-wise read-before-write
@@ -888,22 +948,62 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     double tacc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     sh.tacc = tacc;
     const double tstart = now_us();
-    int sweeps = o.ortho == 1 ? chol_setup(M, sh, &status) : ortho_setup(M, sh, &status);
+    // warm start from the state a previous keep_state call left (its record's
+    // status is still in `out`): re-orthonormalised orbitals, kept potential
+    bool warm = o.warm_start != 0 && out[XTB_OUT_STATUS] == 0.0;
+    __syncthreads();  // every thread has read the previous status
+    if (warm) warm = warm_orthonormalise(M, sh);
+    int sweeps = 0;
+    if (!warm) sweeps = o.ortho == 1 ? chol_setup(M, sh, &status) : ortho_setup(M, sh, &status);
     tacc[0] = now_us() - tstart;
     double ef[2] = {0.0, 0.0};
 
-    // guess: zero charges -> zero potential (scf/base.py:363-420)
-    for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = 0.0;
-    __syncthreads();
-    Mixer mx{0, 0, o.generations + 1};
+    if (warm) {
+        // orbital-energy estimates of the kept orbitals in the new Fock
+        // matrix, rows sorted by them: the first step can then decouple
+        // occupied from virtual orbitals without a full eigensolve
+        const int n = M.n, w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        fock_build(M, M.pot);
+        gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.F, n, 0.0, M.T, n, sh.big);
+        for (int i = w; i < n; i += NWAVE) {
+            double d = 0.0;
+            for (int c = lane; c < n; c += WAVE) d = fma(M.T[(size_t)i * n + c], M.Ct[(size_t)i * n + c], d);
+            d = wave_sum(d);
+            if (lane == 0) sh.ev[i] = d;
+        }
+        __syncthreads();
+        for (int j = threadIdx.x; j < sh.npow; j += NT) {
+            sh.skey[j] = (j < n) ? sh.ev[j] : INFINITY;
+            sh.sidx[j] = j;
+        }
+        __syncthreads();
+        wg_bitonic_sort(sh.skey, sh.sidx, sh.npow);
+        for (int r = w; r < n; r += NWAVE) {
+            const glbf* src = M.Ct + (size_t)sh.sidx[r] * n;
+            for (int c = lane; c < n; c += WAVE) M.F[(size_t)r * n + c] = src[c];
+        }
+        __syncthreads();
+        for (int r = w; r < n; r += NWAVE)
+            for (int c = lane; c < n; c += WAVE) M.Ct[(size_t)r * n + c] = M.F[(size_t)r * n + c];
+        __syncthreads();
+        sh.tacc[7] = 1.0;  // virtual-orbital estimates are fresh
+    } else {
+        // guess: zero charges -> zero potential (scf/base.py:363-420)
+        for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = 0.0;
+        __syncthreads();
+    }
+    Mixer mx{0, 0, o.generations + 1, !warm};
     int iters = 0;
     // unrolling/default.py:80-100: evaluate the guess, optionally mix it
     // Inexact SCF: each step's eigensolve is converged only as far as the
     // current SCF residual warrants (1e-2 of the last |dv|_inf); the last
     // step, from which every reported quantity comes, to full tolerance.
     double resid = 1e-2;
-    const int sw_first = scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef,
-                                  o.eig_inexact > 0.0 ? o.first_tol : 0.0, false, true);
+    // first step: a cold start diagonalises to first_tol; a warm start, whose
+    // potential is already close, decouples to 1e-2 of that
+    const double loose0 = o.eig_inexact > 0.0 ? (warm ? 1e-2 * o.first_tol : o.first_tol) : 0.0;
+    const int sw_first = scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, loose0,
+                                  warm && o.eig_inexact > 0.0, true);
     sweeps += sw_first;
     ++iters;
     if (o.mix_guess) {
@@ -925,8 +1025,8 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     // converged_to_charges (scf/base.py:472-513): density of the last SCF
     // output potential; the potential of its charges goes to st.pot
     int kdec = 0;  // > 0: the final step stopped at the decoupling, k occupied rows first
-    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false, false,
-                                 o.eig_inexact > 0.0, &kdec);
+    const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false,
+                                 o.keep_state != 0, o.eig_inexact > 0.0, &kdec);
     sweeps += sw_last;
 
     // energies (scf/base.py:514-623)
@@ -963,12 +1063,30 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         fock_build(M, M.vnew);
         gemm<false, false>(k, n, n, 1.0, M.Ct, n, M.F, n, 0.0, M.T, n, sh.big);
         gemm<false, true>(k, k, n, 1.0, M.T, n, M.Ct, n, 0.0, M.F, k, sh.big);
+        // occupation of row i: its index before the final sort when rows were kept sorted
+        auto focc = [&](int i) { return sh.ftot[o.keep_state ? sh.sidx[i] : i]; };
         for (int i = w; i < k; i += NWAVE)
-            for (int j = lane; j < k; j += WAVE) M.F[(size_t)i * k + j] *= 0.5 * (sh.ftot[i] + sh.ftot[j]);
+            for (int j = lane; j < k; j += WAVE) M.F[(size_t)i * k + j] *= 0.5 * (focc(i) + focc(j));
         __syncthreads();
         glbf* R = M.F + (size_t)k * k;
         gemm<false, false>(k, n, k, 1.0, M.F, k, M.Ct, n, 0.0, R, n, sh.big);
         gemm<true, false>(n, n, k, 1.0, M.Ct, n, R, n, 0.0, M.T, n, sh.big);
+    } else if (o.want_w && o.keep_state) {
+        // W = C diag(f e) C^T (analytical.py:186-190) with the orbitals kept:
+        // their rows are in ascending order (row j was row sidx[j]) and the
+        // occupied ones a prefix; f e C goes to F
+        const int n = M.n, w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
+        int kk = 0;
+        for (int j = threadIdx.x; j < n; j += NT)
+            if (sh.ftot[sh.sidx[j]] > 0.0) kk = j + 1;
+        const int kocc = (int)wg_max((double)kk, sh.red);
+        for (int j = w; j < kocc; j += NWAVE) {
+            const int o0 = sh.sidx[j];
+            const double f = sh.ftot[o0] * sh.ev[o0];
+            for (int c = lane; c < n; c += WAVE) M.F[(size_t)j * n + c] = f * M.Ct[(size_t)j * n + c];
+        }
+        __syncthreads();
+        gemm<true, false>(n, n, kocc, 1.0, M.Ct, n, M.F, n, 0.0, M.T, n, sh.big);
     } else if (o.want_w) {
         // W = C diag(f e) C^T (analytical.py:186-190) over the occupied
         // orbitals; the orbitals are not needed afterwards, so Ct takes f e C
@@ -993,6 +1111,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_T_EIGEN] = tacc[2];
         out[XTB_OUT_T_TOTAL] = now_us() - tstart;
         out[XTB_OUT_T_START] = tstart;
+        out[XTB_OUT_T_PD] = tacc[3];
         out[XTB_OUT_T_BSUB] = tacc[4];
         out[XTB_OUT_T_BROW] = tacc[5];
         out[XTB_OUT_PD_STEPS] = tacc[6];

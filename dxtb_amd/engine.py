@@ -206,8 +206,12 @@ class ScfOptions:
     # gapped closed shells stop at the occupied-virtual decoupling, which
     # fixes the density, energies and W but not the orbital energies
     orbital_energies: bool = False
+    # molecular dynamics / optimisation: keep each molecule's orbitals and
+    # potential in the pipeline and start the next call on the same batch
+    # from them (new positions); same fixed point, fewer iterations
+    warm_start: bool = False
 
-    def to_struct(self, want_w: bool, padded: bool) -> _lib.XtbScfOpts:
+    def to_struct(self, want_w: bool, padded: bool, warm: bool = False) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
         return _lib.XtbScfOpts(
             maxiter=self.maxiter, mix_guess=int(self.mix_guess), generations=self.generations,
@@ -218,7 +222,8 @@ class ScfOptions:
             damp=self.damp, damp_init=self.damp_init, diag_offset=self.diag_offset,
             fermi_thresh=thr, jacobi_tol=self.jacobi_tol, bd_gap_kt=self.bd_gap_kt,
             pd_kmax=self.pd_kmax, first_tol=self.first_tol, ortho=self.ortho,
-            orbitals=int(self.orbital_energies),
+            orbitals=int(self.orbital_energies), keep_state=int(self.warm_start),
+            warm_start=int(self.warm_start and warm),
         )
 
 
@@ -252,6 +257,7 @@ class Pipeline:
         self.occ = z(2 * nv)
         self.qsh = z(int(L.sh0[-1]))
         self.out = torch.zeros(L.B * _lib.OUT_STRIDE, dtype=torch.float64, device=dev)
+        self._kept = False  # the state buffers hold orbitals and potential to warm-start from
         self.grad = z(3 * natt)
         self.dedcn = z(natt)
         self.hist_x = self.hist_f = None
@@ -301,7 +307,10 @@ class Pipeline:
                                       self.cn.data_ptr(), s), "xtb_build_s_h0")
         _lib.check(lib.xtb_build_gamma(b, p, pos, self.gamma.data_ptr(), s), "xtb_build_gamma")
         _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
-        o = opts.to_struct(want_w=forces, padded=self.L.padded)
+        # a warm start needs the state a previous keep_state launch on these
+        # buffers left (in stream order, so the launches need no other sync)
+        o = opts.to_struct(want_w=forces, padded=self.L.padded, warm=self._kept)
+        self._kept = False
         self.state = self._state()
         if timing:
             ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -312,6 +321,7 @@ class Pipeline:
         if timing:
             ev1.record(st)
             self.scf_events = (ev0, ev1)
+        self._kept = bool(opts.warm_start)
         if forces:
             self.run_forces(stream)
 

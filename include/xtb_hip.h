@@ -92,6 +92,7 @@ extern "C" {
 #define XTB_OUT_SWEEPS_FIRST 18  /* Jacobi sweeps of the first SCF step */
 #define XTB_OUT_SWEEPS_LAST 19   /* Jacobi sweeps of the final (exact) step */
 #define XTB_OUT_T_START 20  /* device wall clock at the molecule's start, microseconds */
+#define XTB_OUT_T_PD 21     /* perturbative decoupling steps (part of T_EIGEN) */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {
@@ -193,6 +194,16 @@ typedef struct xtb_scf_opts {
                                 decoupled (density, energies and W are the same),
                                 emo is then NaN and the Fermi levels are mid-gap
                                 values of the decoupled blocks' diagonals */
+    int32_t keep_state;      /* 1: leave the orbitals (C, rows in ascending energy
+                                order) and the final potential (pot) in the state
+                                buffers for a following warm start */
+    int32_t warm_start;      /* 1: start from what a previous keep_state call on
+                                the same batch and state buffers left there (new
+                                positions allowed): the kept orbitals are
+                                re-orthonormalised against the new overlap, the
+                                SCF starts from the kept potential with a fresh
+                                mixer.  Molecules whose previous status was not
+                                0 start cold.  Converges to the same fixed point */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */

@@ -25,6 +25,23 @@ def run_native(mols, padded=False, forces=False, chrg=None, **opt):
     return pipe
 
 
+def native_pipeline(mols, padded=False):
+    """One pipeline for a fixed set of molecules, and a function that runs it
+    at new positions: (pipe, run(mols, opts, forces))."""
+    dev = torch.device("cuda:0")
+    zs = np.unique(np.concatenate([z for z, _ in mols]))
+    tb = SpeciesTables(zs)
+    lay = BatchLayout([np.asarray(z) for z, _ in mols], tb, padded=padded, chrg=np.zeros(len(mols)))
+    pipe = Pipeline(lay, DeviceTables(tb, dev), dev)
+
+    def run(ms, opts, forces=False):
+        pos = torch.tensor(np.concatenate([p for _, p in ms]), dtype=torch.float64, device=dev)
+        pipe.run(pos, opts, forces=forces)
+        torch.cuda.synchronize()
+
+    return pipe, run
+
+
 def block(pipe, buf, b):
     """Real-orbital block of a packed matrix buffer for molecule b (numpy)."""
     m = pipe.matrix(buf, b).cpu().numpy()
