@@ -119,3 +119,17 @@ def test_purification_projector_and_gap_estimates():
         lumo = lanczos_extreme(F, eye - D, largest=False)
         assert homo <= float(w[nocc - 1]) + 1e-12 and homo > float(w[nocc - 1]) - 1e-4
         assert lumo >= float(w[nocc]) - 1e-12 and lumo < float(w[nocc]) + 1e-4
+
+
+def test_warm_start_flags():
+    """A warm-start run keeps its state; it starts warm only when the
+    pipeline holds a kept state from the previous run."""
+    from dxtb_amd.engine import ScfOptions
+
+    o = ScfOptions(warm_start=True)
+    s = o.to_struct(want_w=True, padded=False, warm=False)
+    assert (s.keep_state, s.warm_start) == (1, 0)
+    s = o.to_struct(want_w=True, padded=False, warm=True)
+    assert (s.keep_state, s.warm_start) == (1, 1)
+    s = ScfOptions().to_struct(want_w=False, padded=False, warm=True)
+    assert (s.keep_state, s.warm_start) == (0, 0)
