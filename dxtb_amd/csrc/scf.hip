@@ -68,7 +68,16 @@ struct Shared {
     ldsi* sidx;
     int npow, big_cap;
     double* tacc;  // per-thread stage timers (meaningful on thread 0)
+#ifdef XTB_PD_TIMERS
+    double* ptacc;  // phases of the perturbative step (instrumented builds only)
+#endif
 };
+
+#ifdef XTB_PD_TIMERS
+#define PD_MARK(sh, i, t) do { const double t_ = now_us(); (sh).ptacc[i] += t_ - (t); (t) = t_; } while (0)
+#else
+#define PD_MARK(sh, i, t) do { } while (0)
+#endif
 
 // device wall clock (constant 100 MHz) in microseconds
 __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
@@ -341,7 +350,7 @@ __device__ __noinline__ bool warm_orthonormalise(const Mol& M, const Shared& sh)
     glbf* R = M.F;
     glbf* Dg = M.T;
     gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.S, n, 0.0, M.T, n, sh.big);  // C S
-    gemm<false, true>(n, n, n, 1.0, M.T, n, M.Ct, n, 0.0, G, n, sh.big);     // C S C^T
+    gemm_sym<false, true>(n, n, 1.0, M.T, n, M.Ct, n, G, n, sh.big);         // C S C^T
     if (!chol_factor(G, n, R, Dg, sh)) return false;
     for (int i0 = 0; i0 < n; i0 += NB) {
         const int nb = n - i0 < NB ? n - i0 : NB;
@@ -501,11 +510,15 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
     const bool refresh = sh.tacc[7] == 0.0;
     sh.tacc[7] = refresh ? 1.0 : 0.0;
     const int rows = refresh ? n : k;
+    double tm = now_us();
+    (void)tm;
     for (int pass = 0; pass < 2; ++pass) {
         if (pass > 0) fock_build(M, vin);
         // T = C F (rows: all, or the occupied ones); A_ov = T_occ C_vir^T
         gemm<false, false>(rows, n, n, 1.0, Ct, n, M.F, n, 0.0, M.T, n, sh.big);
+        PD_MARK(sh, 0, tm);
         gemm<false, true>(k, m, n, 1.0, M.T, n, Cv, n, 0.0, Kb, m, sh.big);
+        PD_MARK(sh, 1, tm);
         for (int i = w; i < rows; i += NWAVE) {
             double d = 0.0;
             for (int c = lane; c < n; c += WAVE) d = fma(M.T[(size_t)i * n + c], Ct[(size_t)i * n + c], d);
@@ -536,11 +549,14 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         dmin = -wg_max(-dmin, sh.red);
         if (aov <= tol) return 1;  // decoupled: orbitals stay
         const int fail = pass == 0 ? 0 : -1;
+        PD_MARK(sh, 2, tm);
         if (!(dmin > 0.0) || !(kmax <= o.pd_kmax)) return fail;
         glbf* X = M.T;
         glbf* Z = M.T + (size_t)k * k;
-        gemm<false, true>(k, k, m, 1.0, Kb, m, Kb, m, 0.0, X, k, sh.big);
+        gemm_sym<false, true>(k, m, 1.0, Kb, m, Kb, m, X, k, sh.big);
+        PD_MARK(sh, 3, tm);
         gemm<false, false>(k, n, m, 1.0, Kb, m, Cv, n, 0.0, Z, n, sh.big);
+        PD_MARK(sh, 4, tm);
         double xf = 0.0;
         for (int e = threadIdx.x; e < k * k; e += NT) xf = fma(X[e], X[e], xf);
         xf = sqrt(wg_sum(xf, sh.red));
@@ -573,7 +589,9 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
             }
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
+            PD_MARK(sh, 6, tm);
             gemm<false, false>(k, n, k, 1.0, X, k, acc, n, 0.0, tmp, n, sh.big);
+            PD_MARK(sh, 5, tm);
             for (int i = w; i < k; i += NWAVE)
                 for (int c = lane; c < n; c += WAVE) {
                     const size_t e = (size_t)i * n + c;
@@ -581,8 +599,10 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
                 }
             __syncthreads();
         }
+        PD_MARK(sh, 6, tm);
         // C_vir += K^T R
         gemm<true, false>(m, n, k, 1.0, Kb, m, acc, n, 1.0, Cv, n, sh.big);
+        PD_MARK(sh, 7, tm);
         // C_occ <- cos(X) C_occ - sinc(X) Z
         for (int i = w; i < k; i += NWAVE)
             for (int c = lane; c < n; c += WAVE) {
@@ -591,7 +611,9 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
             }
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
+            PD_MARK(sh, 6, tm);
             gemm<false, false>(k, n, k, 1.0, X, k, acc, n, 0.0, tmp, n, sh.big);
+            PD_MARK(sh, 5, tm);
             for (int i = w; i < k; i += NWAVE)
                 for (int c = lane; c < n; c += WAVE) {
                     const size_t e = (size_t)i * n + c;
@@ -602,6 +624,10 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         for (int i = w; i < k; i += NWAVE)
             for (int c = lane; c < n; c += WAVE) Ct[(size_t)i * n + c] = acc[(size_t)i * n + c];
         __syncthreads();
+        PD_MARK(sh, 6, tm);
+#ifdef XTB_PD_TIMERS
+        sh.ptacc[9] += (double)J;
+#endif
         if (kmax * aov <= tol) return 1;  // second-order remainder below the tolerance
     }
     // two passes: good enough for an inexact step; an exact one (loose = 0)
@@ -669,7 +695,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     } else {
         // T = F C ; A = C^T T  -> F
         gemm<false, true>(n, n, n, 1.0, M.F, n, M.Ct, n, 0.0, M.T, n, sh.big);
-        gemm<false, false>(n, n, n, 1.0, M.Ct, n, M.T, n, 0.0, M.F, n, sh.big);
+        gemm_sym<false, false>(n, n, 1.0, M.Ct, n, M.T, n, M.F, n, sh.big);
         double t1 = now_us();
         sw = eig_step(M, sh, o.jacobi_max_sweeps, o.jacobi_tol, loose, o.jacobi_inner, cb_lo, cb_hi, intra);
         sh.tacc[7] = 1.0;  // eigenvalues fresh for the next perturbative step
@@ -695,7 +721,10 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     // P = C diag(f) C^T over the occupied orbitals (F and T are free here)
     t0 = now_us();
     const int kocc = gather_occupied(M, sh, M.F, M.T, false, sort_rows);
-    gemm<true, false>(n, n, kocc, 1.0, M.F, n, M.T, n, 0.0, M.P, n, sh.big);
+    gemm_sym<true, false>(n, kocc, 1.0, M.F, n, M.T, n, M.P, n, sh.big);
+#ifdef XTB_PD_TIMERS
+    sh.ptacc[8] += now_us() - t0;
+#endif
     if (sort_rows) {
         // the next step starts from the orbitals in ascending energy order
         for (int r = w; r < n; r += NWAVE)
@@ -947,6 +976,10 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     int status = 0;
     double tacc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     sh.tacc = tacc;
+#ifdef XTB_PD_TIMERS
+    double ptacc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    sh.ptacc = ptacc;
+#endif
     const double tstart = now_us();
     // warm start from the state a previous keep_state call left (its record's
     // status is still in `out`): re-orthonormalised orbitals, kept potential
@@ -1070,7 +1103,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         __syncthreads();
         glbf* R = M.F + (size_t)k * k;
         gemm<false, false>(k, n, k, 1.0, M.F, k, M.Ct, n, 0.0, R, n, sh.big);
-        gemm<true, false>(n, n, k, 1.0, M.Ct, n, R, n, 0.0, M.T, n, sh.big);
+        gemm_sym<true, false>(n, k, 1.0, M.Ct, n, R, n, M.T, n, sh.big);
     } else if (o.want_w && o.keep_state) {
         // W = C diag(f e) C^T (analytical.py:186-190) with the orbitals kept:
         // their rows are in ascending order (row j was row sidx[j]) and the
@@ -1086,12 +1119,12 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
             for (int c = lane; c < n; c += WAVE) M.F[(size_t)j * n + c] = f * M.Ct[(size_t)j * n + c];
         }
         __syncthreads();
-        gemm<true, false>(n, n, kocc, 1.0, M.Ct, n, M.F, n, 0.0, M.T, n, sh.big);
+        gemm_sym<true, false>(n, kocc, 1.0, M.Ct, n, M.F, n, M.T, n, sh.big);
     } else if (o.want_w) {
         // W = C diag(f e) C^T (analytical.py:186-190) over the occupied
         // orbitals; the orbitals are not needed afterwards, so Ct takes f e C
         const int kocc = gather_occupied(M, sh, M.F, M.Ct, true);
-        gemm<true, false>(M.n, M.n, kocc, 1.0, M.F, M.n, M.Ct, M.n, 0.0, M.T, M.n, sh.big);
+        gemm_sym<true, false>(M.n, kocc, 1.0, M.F, M.n, M.Ct, M.n, M.T, M.n, sh.big);
     }
     if (threadIdx.x == 0) {
         const double escf = acc[0] + acc[1] + acc[2];
@@ -1112,6 +1145,9 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_T_TOTAL] = now_us() - tstart;
         out[XTB_OUT_T_START] = tstart;
         out[XTB_OUT_T_PD] = tacc[3];
+#ifdef XTB_PD_TIMERS
+        for (int i = 0; i < 10; ++i) out[22 + i] = ptacc[i];
+#endif
         out[XTB_OUT_T_BSUB] = tacc[4];
         out[XTB_OUT_T_BROW] = tacc[5];
         out[XTB_OUT_PD_STEPS] = tacc[6];

@@ -116,10 +116,12 @@ __device__ inline v4d v4zero() {
 // barrier per panel separates the stages.  `lds` holds >= 4*KP*(TILE+1)
 // doubles.  Each global load instruction covers contiguous memory: lanes run
 // along m (or n) when the operand is stored that way, along k otherwise.
+// With `upper` (M == N, a product known to be symmetric) only the tiles on
+// and above the diagonal are formed; wg_mirror_lower fills the rest.
 template <bool TA, bool TB, int TILE, int KP>
 __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
                                           int lda, const glbf* __restrict__ B, int ldb, double beta,
-                                          glbf* __restrict__ C, int ldc, ldsf* lds) {
+                                          glbf* __restrict__ C, int ldc, ldsf* lds, int upper) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
     constexpr int LDT = TILE + 1;
     constexpr int E = TILE * KP / NT;  // staged elements per thread and operand
@@ -151,7 +153,9 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
         if (TB) { k = idx % KP; n = idx / KP; }       // B[n][k]: contiguous k
         else { n = idx % TILE; k = idx / TILE; }      // B[k][n]: contiguous n
     };
+    upper = __builtin_amdgcn_readfirstlane(upper);
     for (int tile = 0; tile < tm * tn; ++tile) {
+        if (upper && tile / tn > tile % tn) continue;
         const int m0 = (tile / tn) * TILE, n0 = (tile % tn) * TILE;
         // 16x16 fragments of this wave that hold any output (the last tile
         // row/column is partial): the others skip their MFMAs
@@ -269,11 +273,43 @@ constexpr int GEMM_KP = XTB_GEMM_KP;
 constexpr int NGEMM_MFMA_LDS = 4 * GEMM_KP * (128 + 1);
 
 // dispatch: matrix-core GEMM with a tile matched to the size
+__device__ inline int gemm_tile(int M, int N) { return (M <= 64 && N <= 64) ? 64 : 128; }
+
 template <bool TA, bool TB>
 __device__ inline void gemm(int M, int N, int K, double alpha, const glbf* A, int lda,
                             const glbf* B, int ldb, double beta, glbf* C, int ldc, ldsf* lds) {
-    if (M <= 64 && N <= 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
-    else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds);
+    if (gemm_tile(M, N) == 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0);
+    else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0);
+}
+
+// C (n x n) = alpha op(A) op(B) for a product known to be symmetric: the
+// tiles on and above the diagonal on matrix cores, the others mirrored
+// through LDS in 64 x 64 blocks (coalesced both ways).  `lds` holds
+// >= max(NGEMM_MFMA_LDS, 64 * 65) doubles.
+template <bool TA, bool TB>
+__device__ inline void gemm_sym(int n, int K, double alpha, const glbf* A, int lda, const glbf* B, int ldb,
+                                glbf* C, int ldc, ldsf* lds) {
+    const int tile = gemm_tile(n, n);
+    if (tile == 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(n, n, K, alpha, A, lda, B, ldb, 0.0, C, ldc, lds, 1);
+    else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(n, n, K, alpha, A, lda, B, ldb, 0.0, C, ldc, lds, 1);
+    // 64-blocks (bi, bj) whose tile lies below the diagonal: C_bi,bj = C_bj,bi^T
+    const int nb = (n + 63) / 64, per = tile / 64;
+    for (int bi = 1; bi < nb; ++bi)
+        for (int bj = 0; bj < bi; ++bj) {
+            if (bi / per <= bj / per) continue;  // formed by the GEMM
+            for (int e = threadIdx.x; e < 64 * 64; e += NT) {
+                const int r = e / 64, c = e % 64;  // source row bj*64 + r, column bi*64 + c
+                const int gr = bj * 64 + r, gc = bi * 64 + c;
+                lds[r * 65 + c] = (gr < n && gc < n) ? C[(size_t)gr * ldc + gc] : 0.0;
+            }
+            __syncthreads();
+            for (int e = threadIdx.x; e < 64 * 64; e += NT) {
+                const int c = e / 64, r = e % 64;  // destination row bi*64 + c, column bj*64 + r
+                const int gr = bi * 64 + c, gc = bj * 64 + r;
+                if (gr < n && gc < n) C[(size_t)gr * ldc + gc] = lds[r * 65 + c];
+            }
+            __syncthreads();
+        }
 }
 
 // ---------------------------------------------------------------------------
