@@ -491,8 +491,8 @@ __device__ inline void fock_build(const Mol& M, const glbf* vin) {
 // a series would need more than 9 terms, or (exact calls, loose = 0) two
 // passes leave a coupling above the tolerance.  On return 1, sh.ev holds the
 // diagonal of C F C^T in the rows' order (occupied rows: of the last pass).
-// Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the
-// Horner accumulators.
+// Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the two
+// Horner accumulators (k x n each, k <= n/2).
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_scf_opts& o, int k,
                                     double loose, const glbf* vin) {
@@ -579,50 +579,46 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
                 ph[j] = -sg / (f * (2.0 * j + 1.0) * (2.0 * j + 2.0));
             }
         }
-        glbf* acc = M.F;
-        glbf* tmp = M.F + (size_t)k * n;
-        // R = sinc(X) C_occ + phi(X) Z  ->  acc
+        // Horner steps acc <- X acc + a_j C_occ + b_j Z, the update fused into
+        // the GEMM epilogue; the two k x n halves of F alternate
+        glbf* cur = M.F;
+        glbf* nxt = M.F + (size_t)k * n;
+        // R = sinc(X) C_occ + phi(X) Z
         for (int i = w; i < k; i += NWAVE)
             for (int c = lane; c < n; c += WAVE) {
                 const size_t e = (size_t)i * n + c;
-                acc[e] = ss[J] * Ct[e] + ph[J] * Z[e];
+                cur[e] = ss[J] * Ct[e] + ph[J] * Z[e];
             }
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
-            gemm<false, false>(k, n, k, 1.0, X, k, acc, n, 0.0, tmp, n, sh.big);
+            gemm_epi(k, n, k, X, k, cur, n, nxt, n, Ct, ss[j], Z, ph[j], sh.big);
             PD_MARK(sh, 5, tm);
-            for (int i = w; i < k; i += NWAVE)
-                for (int c = lane; c < n; c += WAVE) {
-                    const size_t e = (size_t)i * n + c;
-                    acc[e] = tmp[e] + ss[j] * Ct[e] + ph[j] * Z[e];
-                }
-            __syncthreads();
+            glbf* t = cur;
+            cur = nxt;
+            nxt = t;
         }
         PD_MARK(sh, 6, tm);
         // C_vir += K^T R
-        gemm<true, false>(m, n, k, 1.0, Kb, m, acc, n, 1.0, Cv, n, sh.big);
+        gemm<true, false>(m, n, k, 1.0, Kb, m, cur, n, 1.0, Cv, n, sh.big);
         PD_MARK(sh, 7, tm);
         // C_occ <- cos(X) C_occ - sinc(X) Z
         for (int i = w; i < k; i += NWAVE)
             for (int c = lane; c < n; c += WAVE) {
                 const size_t e = (size_t)i * n + c;
-                acc[e] = cc[J] * Ct[e] - ss[J] * Z[e];
+                cur[e] = cc[J] * Ct[e] - ss[J] * Z[e];
             }
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
-            gemm<false, false>(k, n, k, 1.0, X, k, acc, n, 0.0, tmp, n, sh.big);
+            gemm_epi(k, n, k, X, k, cur, n, nxt, n, Ct, cc[j], Z, -ss[j], sh.big);
             PD_MARK(sh, 5, tm);
-            for (int i = w; i < k; i += NWAVE)
-                for (int c = lane; c < n; c += WAVE) {
-                    const size_t e = (size_t)i * n + c;
-                    acc[e] = tmp[e] + cc[j] * Ct[e] - ss[j] * Z[e];
-                }
-            __syncthreads();
+            glbf* t = cur;
+            cur = nxt;
+            nxt = t;
         }
         for (int i = w; i < k; i += NWAVE)
-            for (int c = lane; c < n; c += WAVE) Ct[(size_t)i * n + c] = acc[(size_t)i * n + c];
+            for (int c = lane; c < n; c += WAVE) Ct[(size_t)i * n + c] = cur[(size_t)i * n + c];
         __syncthreads();
         PD_MARK(sh, 6, tm);
 #ifdef XTB_PD_TIMERS

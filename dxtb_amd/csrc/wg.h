@@ -116,12 +116,23 @@ __device__ inline v4d v4zero() {
 // barrier per panel separates the stages.  `lds` holds >= 4*KP*(TILE+1)
 // doubles.  Each global load instruction covers contiguous memory: lanes run
 // along m (or n) when the operand is stored that way, along k otherwise.
+// Panel loop per fragment pattern (split) or one loop for both: measured per
+// operand layout under a full chip of workgroups.
+#ifndef XTB_GEMM_SPLIT
+#define XTB_GEMM_SPLIT(TA, TB) (true)
+#endif
+
 // With `upper` (M == N, a product known to be symmetric) only the tiles on
 // and above the diagonal are formed; wg_mirror_lower fills the rest.
-template <bool TA, bool TB, int TILE, int KP>
+// With EPI, C = alpha op(A) op(B) + s1 E1 + s2 E2 (E1, E2 with C's leading
+// dimension; beta unused): an elementwise update fused into the epilogue.
+template <bool TA, bool TB, int TILE, int KP, bool EPI = false>
 __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
                                           int lda, const glbf* __restrict__ B, int ldb, double beta,
-                                          glbf* __restrict__ C, int ldc, ldsf* lds, int upper) {
+                                          glbf* __restrict__ C, int ldc, ldsf* lds, int upper,
+                                          const glbf* __restrict__ E1 = nullptr,
+                                          const glbf* __restrict__ E2 = nullptr, double s1 = 0.0,
+                                          double s2 = 0.0) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
     constexpr int LDT = TILE + 1;
     constexpr int E = TILE * KP / NT;  // staged elements per thread and operand
@@ -240,13 +251,35 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
             store(lds);
             __syncthreads();
         }
-        for (int p = 0; p < np; ++p) {
-            ldsf* cur = lds + (p & 1) * STAGE;
-            if (p + 1 < np) fetch((p + 1) * KP);
-            if (full) panel_full(cur, cur + KP * LDT);
-            else panel_part(cur, cur + KP * LDT);
-            if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
-            __syncthreads();
+        if (XTB_GEMM_SPLIT(TA, TB)) {
+            // one panel loop per fragment pattern, so the accumulators are
+            // not merged between the two code paths after every panel
+            if (full) {
+                for (int p = 0; p < np; ++p) {
+                    ldsf* cur = lds + (p & 1) * STAGE;
+                    if (p + 1 < np) fetch((p + 1) * KP);
+                    panel_full(cur, cur + KP * LDT);
+                    if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
+                    __syncthreads();
+                }
+            } else {
+                for (int p = 0; p < np; ++p) {
+                    ldsf* cur = lds + (p & 1) * STAGE;
+                    if (p + 1 < np) fetch((p + 1) * KP);
+                    panel_part(cur, cur + KP * LDT);
+                    if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
+                    __syncthreads();
+                }
+            }
+        } else {
+            for (int p = 0; p < np; ++p) {
+                ldsf* cur = lds + (p & 1) * STAGE;
+                if (p + 1 < np) fetch((p + 1) * KP);
+                if (full) panel_full(cur, cur + KP * LDT);
+                else panel_part(cur, cur + KP * LDT);
+                if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
+                __syncthreads();
+            }
         }
 #pragma unroll
         for (int i = 0; i < FM; ++i)
@@ -257,8 +290,9 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                     const int gm = m0 + wm * (TILE / 2) + i * 16 + lg + 4 * r;
                     const int gn = n0 + wn * (TILE / 4) + j * 16 + li;
                     if (gm < M && gn < N) {
-                        glbf* c = C + (size_t)gm * ldc + gn;
-                        *c = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * *c);
+                        const size_t e = (size_t)gm * ldc + gn;
+                        if (EPI) C[e] = fma(alpha, acc[i][j][r], fma(s1, E1[e], s2 * E2[e]));
+                        else C[e] = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * C[e]);
                     }
                 }
     }
@@ -280,6 +314,16 @@ __device__ inline void gemm(int M, int N, int K, double alpha, const glbf* A, in
                             const glbf* B, int ldb, double beta, glbf* C, int ldc, ldsf* lds) {
     if (gemm_tile(M, N) == 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0);
     else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0);
+}
+
+// C = A B + s1 E1 + s2 E2 (row-major, E1 and E2 with C's leading dimension)
+__device__ inline void gemm_epi(int M, int N, int K, const glbf* A, int lda, const glbf* B, int ldb,
+                                glbf* C, int ldc, const glbf* E1, double s1, const glbf* E2, double s2,
+                                ldsf* lds) {
+    if (gemm_tile(M, N) == 64)
+        wg_gemm_mfma<false, false, 64, GEMM_KP, true>(M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, lds, 0, E1, E2, s1, s2);
+    else
+        wg_gemm_mfma<false, false, 128, GEMM_KP, true>(M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, lds, 0, E1, E2, s1, s2);
 }
 
 // C (n x n) = alpha op(A) op(B) for a product known to be symmetric: the
