@@ -481,7 +481,8 @@ __device__ inline void fock_build(const Mol& M, const glbf* vin) {
 //   C_vir <- C_vir + K^T [sinc(sqrt X) C_occ + phi(X) Z]
 // with X = K K^T (k x k), Z = K C_vir (k x n), phi(x) = (cos sqrt x - 1)/x;
 // the functions of X are power series evaluated by Horner's rule, truncated
-// once the next term is below 1e-17 (||X||_F bounds the spectral norm).  The
+// once the next term is below 1e-17, or 1e-13 in inexact steps (||X||_F
+// bounds the spectral norm).  The
 // coupling left is second order, about max|K| max|A_ov|; a second pass runs
 // when that estimate exceeds the tolerance.
 // Returns 1 when done (rotated, or already decoupled to the tolerance), 0 to
@@ -560,10 +561,15 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         double xf = 0.0;
         for (int e = threadIdx.x; e < k * k; e += NT) xf = fma(X[e], X[e], xf);
         xf = sqrt(wg_sum(xf, sh.red));
-        // highest power J: first omitted cosine term ||X||^(J+1) / (2J+2)! < 1e-17
-        int J = 1;
-        double term = xf * xf / 24.0;
-        while (term > 1e-17) {
+        // highest power J kept: the first omitted cosine term (the largest of
+        // the three series) ||X||^(J+1) / (2J+2)! is below 1e-17 in an exact
+        // step and below min(1e-13, 1e-3 loose) in an inexact one.  What is
+        // left out perturbs orthonormality by that much; over an SCF run it
+        // adds up to ~1e-12, far below the results' tolerances
+        const double tolj = loose > 0.0 ? fmin(1e-13, fmax(1e-17, 1e-3 * loose)) : 1e-17;
+        int J = 0;
+        double term = 0.5 * xf;
+        while (term > tolj) {
             ++J;
             term *= xf / ((2.0 * J + 1.0) * (2.0 * J + 2.0));
             if (J > 8) return fail;  // F and Ct untouched in this pass

@@ -256,3 +256,27 @@ def test_warm_start_along_a_trajectory(fx):
     c = run_native(geo[2], forces=True)
     assert np.array_equal(pipe.records()[:, 5].cpu().numpy(), c.records()[:, 5].cpu().numpy())
     assert np.abs(pipe.energies().cpu().numpy() - c.energies().cpu().numpy()).max() < 1e-12
+
+
+def test_kept_orbitals_stay_orthonormal(fx):
+    """The intermediate steps' rotations truncate their series at 1e-13 (the
+    final step at 1e-17) and are never re-orthonormalised: after a full SCF
+    the orbitals kept for a warm start must still satisfy C S C^T = I to
+    1e-11, and the density must hold the electron count to 1e-10."""
+    from helpers import TIGHT, native_pipeline
+
+    rng = np.random.default_rng(13)
+    mols = [W.water_cluster(rng, 60), W.chain(rng, 60), mol(fx, "lys_xao")]
+    print("case: orthonormality of the kept orbitals", flush=True)
+    pipe, run = native_pipeline(mols)
+    run(mols, ScfOptions(**TIGHT, warm_start=True), forces=False)
+    assert int(pipe.records()[:, 6].abs().max().item()) == 0
+    for b in range(len(mols)):
+        C = pipe.matrix(pipe.C, b)
+        S = pipe.matrix(pipe.S, b)
+        G = C @ S @ C.T
+        err = (G - torch.eye(G.shape[0], dtype=G.dtype, device=G.device)).abs().max().item()
+        assert err < 1e-11, (b, err)
+        P = pipe.matrix(pipe.P, b)
+        nel = float(pipe.L.nel_a[b] + pipe.L.nel_b[b])
+        assert abs((P * S).sum().item() - nel) < 1e-10
