@@ -503,13 +503,18 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
     glbf* Ct = M.Ct;
     glbf* Cv = M.Ct + (size_t)k * n;
     glbf* Kb = M.P;  // k x m, leading dimension m
-    // Virtual orbital energies only scale the amplitudes: an error of one
-    // SCF step in them leaves a second-order coupling, so every other
-    // perturbative step reuses the previous step's values (sh.skey, in the
-    // rows' order) and forms only the occupied rows of C F.
-    // sh.tacc[7] = 1 marks the virtual energies as fresh.
-    const bool refresh = sh.tacc[7] == 0.0;
-    sh.tacc[7] = refresh ? 1.0 : 0.0;
+    // Virtual orbital energies only scale the amplitudes: an error of a few
+    // SCF steps in them leaves a second-order coupling, so they are formed
+    // (all rows of C F) every XTB_PD_REFRESH-th perturbative step only; the
+    // others reuse the last values (sh.skey, in the rows' order) and form
+    // the occupied rows of C F.  Every 4th step costs no SCF iterations on
+    // 500-orbital molecules; never refreshing them costs two.
+#ifndef XTB_PD_REFRESH
+#define XTB_PD_REFRESH 4
+#endif
+    // sh.tacc[7] counts the steps since the last refresh (>= 1)
+    const bool refresh = sh.tacc[7] == 0.0 || sh.tacc[7] >= XTB_PD_REFRESH;
+    sh.tacc[7] = refresh ? 1.0 : sh.tacc[7] + 1.0;
     const int rows = refresh ? n : k;
     double tm = now_us();
     (void)tm;
