@@ -649,7 +649,8 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
                         const xtb_params& par, const xtb_scf_opts& o, const glbf* vin,
                         glbf* vout, int* status, double* ef, double loose, bool allow_bd,
-                        bool sort_rows, bool final_step = false, int* decoupled = nullptr) {
+                        bool sort_rows, bool final_step = false, int* decoupled = nullptr,
+                        bool last = false) {
     const int n = M.n;
     // Occupation classes for this step's eigensolve.  With integer alpha/beta
     // counts and frontier gaps of at least bd_gap_kt x kT in the previous
@@ -738,7 +739,8 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
             for (int c = lane; c < n; c += WAVE) M.Ct[(size_t)r * n + c] = M.F[(size_t)r * n + c];
     }
     sh.tacc[1] += now_us() - t0;
-    // Mulliken charges and orbital energies (scf/base.py:764-815), one wave per row
+    // Mulliken charges and orbital energies (scf/base.py:764-815), one wave
+    // per row; the orbital energies only enter the final energy (last step)
     {
         const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
         for (int i = w; i < n; i += NWAVE) {
@@ -747,9 +749,13 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
                 const glbf* pr = M.P + (size_t)i * n;
                 const glbf* sr = M.S + (size_t)i * n;
                 const glbf* hr = M.H0 + (size_t)i * n;
-                for (int j = l; j < M.nao; j += WAVE) {
-                    sq = fma(pr[j], sr[j], sq);
-                    se = fma(pr[j], hr[j], se);
+                if (last) {
+                    for (int j = l; j < M.nao; j += WAVE) {
+                        sq = fma(pr[j], sr[j], sq);
+                        se = fma(pr[j], hr[j], se);
+                    }
+                } else {
+                    for (int j = l; j < M.nao; j += WAVE) sq = fma(pr[j], sr[j], sq);
                 }
             }
             sq = wave_sum(sq);
@@ -1066,7 +1072,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     // output potential; the potential of its charges goes to st.pot
     int kdec = 0;  // > 0: the final step stopped at the decoupling, k occupied rows first
     const int sw_last = scf_step(M, sh, b, par, o, M.vnew, M.pot, &status, ef, 0.0, false,
-                                 o.keep_state != 0, o.eig_inexact > 0.0, &kdec);
+                                 o.keep_state != 0, o.eig_inexact > 0.0, &kdec, true);
     sweeps += sw_last;
 
     // energies (scf/base.py:514-623)

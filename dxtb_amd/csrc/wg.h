@@ -108,13 +108,13 @@ __device__ inline v4d v4zero() {
     return z;
 }
 
-// C = alpha op(A) op(B) + beta C on matrix cores.  Workgroup tile TILE x TILE
-// (TILE = 64 or 128), 8 waves as 2 (rows) x 4 (cols), k panels of KP staged
-// in LDS as As[k][m], Bs[k][n] (row pitch TILE+1: conflict-free transposed
-// stores), two stages: the next panel is fetched into registers before the
-// current panel's MFMAs and stored to the other stage after them, so one
-// barrier per panel separates the stages.  `lds` holds >= 4*KP*(TILE+1)
-// doubles.  Each global load instruction covers contiguous memory: lanes run
+// C = alpha op(A) op(B) + beta C on matrix cores.  Workgroup tile TM x TN =
+// 32 FM x 64 FN, 8 waves as 2 (rows) x 4 (cols) of FM x FN 16x16 fragments
+// each, k panels of KP staged in LDS as As[k][m], Bs[k][n] (row pitches
+// TM+1, TN+1: conflict-free transposed stores), two stages: the next panel
+// is fetched into registers before the current panel's MFMAs and stored to
+// the other stage after them, so one barrier per panel separates the
+// stages.  `lds` holds >= 2*KP*(TM+TN+2) doubles.  Each global load instruction covers contiguous memory: lanes run
 // along m (or n) when the operand is stored that way, along k otherwise.
 // Panel loop per fragment pattern (split) or one loop for both: measured per
 // operand layout under a full chip of workgroups.
@@ -126,7 +126,7 @@ __device__ inline v4d v4zero() {
 // and above the diagonal are formed; wg_mirror_lower fills the rest.
 // With EPI, C = alpha op(A) op(B) + s1 E1 + s2 E2 (E1, E2 with C's leading
 // dimension; beta unused): an elementwise update fused into the epilogue.
-template <bool TA, bool TB, int TILE, int KP, bool EPI = false>
+template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false>
 __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
                                           int lda, const glbf* __restrict__ B, int ldb, double beta,
                                           glbf* __restrict__ C, int ldc, ldsf* lds, int upper,
@@ -134,10 +134,11 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                                           const glbf* __restrict__ E2 = nullptr, double s1 = 0.0,
                                           double s2 = 0.0) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
-    constexpr int LDT = TILE + 1;
-    constexpr int E = TILE * KP / NT;  // staged elements per thread and operand
-    constexpr int FM = TILE / 32, FN = TILE / 64;
-    constexpr int STAGE = 2 * KP * LDT;
+    constexpr int TM = 32 * FM, TN = 64 * FN;
+    constexpr int LDA_S = TM + 1, LDB_S = TN + 1;
+    constexpr int EA = TM * KP / NT, EB = TN * KP / NT;  // staged elements per thread
+    static_assert(EA * NT == TM * KP && EB * NT == TN * KP, "panel split over the threads");
+    constexpr int STAGE = KP * (LDA_S + LDB_S);
     // arguments of an out-of-line call arrive in vector registers; they are
     // uniform, and saying so turns the tile and fragment tests into scalar
     // branches instead of exec-masked code
@@ -152,25 +153,25 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
     const int w = __builtin_amdgcn_readfirstlane(t / WAVE);
     const int lane = t % WAVE, li = lane % 16, lg = lane / 16;
     const int wm = w / 4, wn = w % 4;
-    const int tm = (M + TILE - 1) / TILE, tn = (N + TILE - 1) / TILE;
+    const int tm = (M + TM - 1) / TM, tn = (N + TN - 1) / TN;
     // element e of this thread: (row along the contiguous direction)
     auto a_mk = [&](int e, int& m, int& k) {
         const int idx = t + e * NT;
-        if (TA) { m = idx % TILE; k = idx / TILE; }   // A[k][m]: contiguous m
+        if (TA) { m = idx % TM; k = idx / TM; }       // A[k][m]: contiguous m
         else { k = idx % KP; m = idx / KP; }          // A[m][k]: contiguous k
     };
     auto b_nk = [&](int e, int& n, int& k) {
         const int idx = t + e * NT;
         if (TB) { k = idx % KP; n = idx / KP; }       // B[n][k]: contiguous k
-        else { n = idx % TILE; k = idx / TILE; }      // B[k][n]: contiguous n
+        else { n = idx % TN; k = idx / TN; }          // B[k][n]: contiguous n
     };
     upper = __builtin_amdgcn_readfirstlane(upper);
     for (int tile = 0; tile < tm * tn; ++tile) {
         if (upper && tile / tn > tile % tn) continue;
-        const int m0 = (tile / tn) * TILE, n0 = (tile % tn) * TILE;
+        const int m0 = (tile / tn) * TM, n0 = (tile % tn) * TN;
         // 16x16 fragments of this wave that hold any output (the last tile
         // row/column is partial): the others skip their MFMAs
-        const int rrem = M - (m0 + wm * (TILE / 2)), crem = N - (n0 + wn * (TILE / 4));
+        const int rrem = M - (m0 + wm * (TM / 2)), crem = N - (n0 + wn * (TN / 4));
         const int nfm = rrem <= 0 ? 0 : (rrem >= FM * 16 ? FM : (rrem + 15) / 16);
         const int nfn = crem <= 0 ? 0 : (crem >= FN * 16 ? FN : (crem + 15) / 16);
         const bool full = nfm == FM && nfn == FN;
@@ -179,17 +180,17 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
         for (int i = 0; i < FM; ++i)
 #pragma unroll
             for (int j = 0; j < FN; ++j) acc[i][j] = v4zero();
-        double pa[E], pb[E];
+        double pa[EA], pb[EB];
         auto fetch = [&](int k0) {
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
+            for (int e = 0; e < EA; ++e) {
                 int m, k;
                 a_mk(e, m, k);
                 const int gm = m0 + m, gk = k0 + k;
                 pa[e] = (gm < M && gk < K) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
             }
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
+            for (int e = 0; e < EB; ++e) {
                 int n, k;
                 b_nk(e, n, k);
                 const int gn = n0 + n, gk = k0 + k;
@@ -198,18 +199,18 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
         };
         auto store = [&](ldsf* st) {
             ldsf* As = st;
-            ldsf* Bs = st + KP * LDT;
+            ldsf* Bs = st + KP * LDA_S;
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
+            for (int e = 0; e < EA; ++e) {
                 int m, k;
                 a_mk(e, m, k);
-                As[k * LDT + m] = pa[e];
+                As[k * LDA_S + m] = pa[e];
             }
 #pragma unroll
-            for (int e = 0; e < E; ++e) {
+            for (int e = 0; e < EB; ++e) {
                 int n, k;
                 b_nk(e, n, k);
-                Bs[k * LDT + n] = pb[e];
+                Bs[k * LDB_S + n] = pb[e];
             }
         };
         // MFMAs of one panel when every fragment of the wave holds output
@@ -218,9 +219,9 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
             for (int kk = 0; kk < KP / 4; ++kk) {
                 double a[FM], b[FN];
 #pragma unroll
-                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDT + wm * (TILE / 2) + i * 16 + li];
+                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDA_S + wm * (TM / 2) + i * 16 + li];
 #pragma unroll
-                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDT + wn * (TILE / 4) + j * 16 + li];
+                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDB_S + wn * (TN / 4) + j * 16 + li];
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -235,9 +236,9 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
             for (int kk = 0; kk < KP / 4; ++kk) {
                 double a[FM], b[FN];
 #pragma unroll
-                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDT + wm * (TILE / 2) + i * 16 + li];
+                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDA_S + wm * (TM / 2) + i * 16 + li];
 #pragma unroll
-                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDT + wn * (TILE / 4) + j * 16 + li];
+                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDB_S + wn * (TN / 4) + j * 16 + li];
 #pragma unroll
                 for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -258,7 +259,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                 for (int p = 0; p < np; ++p) {
                     ldsf* cur = lds + (p & 1) * STAGE;
                     if (p + 1 < np) fetch((p + 1) * KP);
-                    panel_full(cur, cur + KP * LDT);
+                    panel_full(cur, cur + KP * LDA_S);
                     if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
                     __syncthreads();
                 }
@@ -266,7 +267,7 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
                 for (int p = 0; p < np; ++p) {
                     ldsf* cur = lds + (p & 1) * STAGE;
                     if (p + 1 < np) fetch((p + 1) * KP);
-                    panel_part(cur, cur + KP * LDT);
+                    panel_part(cur, cur + KP * LDA_S);
                     if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
                     __syncthreads();
                 }
@@ -275,8 +276,8 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
             for (int p = 0; p < np; ++p) {
                 ldsf* cur = lds + (p & 1) * STAGE;
                 if (p + 1 < np) fetch((p + 1) * KP);
-                if (full) panel_full(cur, cur + KP * LDT);
-                else panel_part(cur, cur + KP * LDT);
+                if (full) panel_full(cur, cur + KP * LDA_S);
+                else panel_part(cur, cur + KP * LDA_S);
                 if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
                 __syncthreads();
             }
@@ -287,8 +288,8 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
             for (int j = 0; j < FN; ++j)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int gm = m0 + wm * (TILE / 2) + i * 16 + lg + 4 * r;
-                    const int gn = n0 + wn * (TILE / 4) + j * 16 + li;
+                    const int gm = m0 + wm * (TM / 2) + i * 16 + lg + 4 * r;
+                    const int gn = n0 + wn * (TN / 4) + j * 16 + li;
                     if (gm < M && gn < N) {
                         const size_t e = (size_t)gm * ldc + gn;
                         if (EPI) C[e] = fma(alpha, acc[i][j][r], fma(s1, E1[e], s2 * E2[e]));
@@ -304,26 +305,51 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
 #define XTB_GEMM_KP 16
 #endif
 constexpr int GEMM_KP = XTB_GEMM_KP;
-constexpr int NGEMM_MFMA_LDS = 4 * GEMM_KP * (128 + 1);
+constexpr int NGEMM_MFMA_LDS = 2 * GEMM_KP * (128 + 128 + 2);  // largest tile, 128 x 128
 
-// dispatch: matrix-core GEMM with a tile matched to the size
-__device__ inline int gemm_tile(int M, int N) { return (M <= 64 && N <= 64) ? 64 : 128; }
+// Tile shape for an M x N product: the least padded area, with one 64 x 64
+// tile's worth of area added per tile for its fixed costs; ties keep the
+// larger tile.  Returns FM * 4 + FN (FM in 2..4 fragments of 32 rows, FN in
+// 1..2 of 64 columns).
+__device__ inline int gemm_shape(int M, int N) {
+    int best = 0;
+    long cost = -1;
+    for (int fm = 4; fm >= 2; --fm)
+        for (int fn = 2; fn >= 1; --fn) {
+            const long tm = (M + 32 * fm - 1) / (32 * fm), tn = (N + 64 * fn - 1) / (64 * fn);
+            const long c = tm * tn * (32L * fm * 64 * fn + 4096);
+            if (cost < 0 || c < cost) { cost = c; best = fm * 4 + fn; }
+        }
+    return best;
+}
 
+#define XTB_GEMM_SHAPES(CALL)        \
+    switch (gemm_shape(M, N)) {      \
+        case 2 * 4 + 1: CALL(2, 1); break; \
+        case 2 * 4 + 2: CALL(2, 2); break; \
+        case 3 * 4 + 1: CALL(3, 1); break; \
+        case 3 * 4 + 2: CALL(3, 2); break; \
+        case 4 * 4 + 1: CALL(4, 1); break; \
+        default: CALL(4, 2); break;  \
+    }
+
+// dispatch: matrix-core GEMM with a tile shape matched to the size
 template <bool TA, bool TB>
 __device__ inline void gemm(int M, int N, int K, double alpha, const glbf* A, int lda,
                             const glbf* B, int ldb, double beta, glbf* C, int ldc, ldsf* lds) {
-    if (gemm_tile(M, N) == 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0);
-    else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0);
+#define XTB_G(fm, fn) wg_gemm_mfma<TA, TB, fm, fn, GEMM_KP>(M, N, K, alpha, A, lda, B, ldb, beta, C, ldc, lds, 0)
+    XTB_GEMM_SHAPES(XTB_G)
+#undef XTB_G
 }
 
 // C = A B + s1 E1 + s2 E2 (row-major, E1 and E2 with C's leading dimension)
 __device__ inline void gemm_epi(int M, int N, int K, const glbf* A, int lda, const glbf* B, int ldb,
                                 glbf* C, int ldc, const glbf* E1, double s1, const glbf* E2, double s2,
                                 ldsf* lds) {
-    if (gemm_tile(M, N) == 64)
-        wg_gemm_mfma<false, false, 64, GEMM_KP, true>(M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, lds, 0, E1, E2, s1, s2);
-    else
-        wg_gemm_mfma<false, false, 128, GEMM_KP, true>(M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, lds, 0, E1, E2, s1, s2);
+#define XTB_G(fm, fn) \
+    wg_gemm_mfma<false, false, fm, fn, GEMM_KP, true>(M, N, K, 1.0, A, lda, B, ldb, 0.0, C, ldc, lds, 0, E1, E2, s1, s2)
+    XTB_GEMM_SHAPES(XTB_G)
+#undef XTB_G
 }
 
 // C (n x n) = alpha op(A) op(B) for a product known to be symmetric: the
@@ -333,9 +359,10 @@ __device__ inline void gemm_epi(int M, int N, int K, const glbf* A, int lda, con
 template <bool TA, bool TB>
 __device__ inline void gemm_sym(int n, int K, double alpha, const glbf* A, int lda, const glbf* B, int ldb,
                                 glbf* C, int ldc, ldsf* lds) {
-    const int tile = gemm_tile(n, n);
-    if (tile == 64) wg_gemm_mfma<TA, TB, 64, GEMM_KP>(n, n, K, alpha, A, lda, B, ldb, 0.0, C, ldc, lds, 1);
-    else wg_gemm_mfma<TA, TB, 128, GEMM_KP>(n, n, K, alpha, A, lda, B, ldb, 0.0, C, ldc, lds, 1);
+    // square tiles: 64 x 64 for small n, 128 x 128 otherwise
+    const int tile = n <= 64 ? 64 : 128;
+    if (tile == 64) wg_gemm_mfma<TA, TB, 2, 1, GEMM_KP>(n, n, K, alpha, A, lda, B, ldb, 0.0, C, ldc, lds, 1);
+    else wg_gemm_mfma<TA, TB, 4, 2, GEMM_KP>(n, n, K, alpha, A, lda, B, ldb, 0.0, C, ldc, lds, 1);
     // 64-blocks (bi, bj) whose tile lies below the diagonal: C_bi,bj = C_bj,bi^T
     const int nb = (n + 63) / 64, per = tile / 64;
     for (int bi = 1; bi < nb; ++bi)
