@@ -113,6 +113,8 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_repulsion.argtypes = [bp, pp, P, P, P]
     lib.xtb_forces.restype = i32
     lib.xtb_forces.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P, P]
+    lib.xtb_param_grad.restype = i32
+    lib.xtb_param_grad.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P]
     if lib.xtb_abi_version() != 1:
         raise NativeLibraryError("libxtbhip.so ABI version mismatch")
     sizes = (i64 * 4)()
@@ -126,7 +128,7 @@ def load(path: str | os.PathLike | None = None):
 
 EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
-    "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces",
+    "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
 )
 
 

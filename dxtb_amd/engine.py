@@ -334,6 +334,20 @@ class Pipeline:
                                        self.dedcn.data_ptr(), self.grad.data_ptr(), s),
                    "xtb_forces")
 
+    def param_grad(self, stream=None) -> torch.Tensor:
+        """dE/dtheta of the last run's energies for the species tables'
+        parameters: (B, 4 nus + 3 nsp) = level, kcn, shpoly, eta (per unique
+        shell), gam3, zeff, arep (per species), atomic units."""
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        tb = self.dt
+        nus, nsp = int(tb.s.nus), int(tb.s.nsp)
+        g = torch.empty((self.L.B, 4 * nus + 3 * nsp), dtype=torch.float64, device=self.device)
+        _lib.check(self.lib.xtb_param_grad(ct.byref(self.batch), ct.byref(tb.s), self.positions.data_ptr(),
+                                           self.cn.data_ptr(), self.S.data_ptr(), ct.byref(self.state),
+                                           g.data_ptr(), ct.c_void_p(st.cuda_stream)),
+                   "xtb_param_grad")
+        return g
+
     def scf_flops(self) -> float:
         """Algorithmic FP64 flops of the SCF launch (fixed convention): per
         molecule (iterations + 1) x (9 1/3 n^3 eigen + 2 n^2 k density)

@@ -117,6 +117,7 @@ class SpeciesTables:
         self.sp_zeff = np.array([els[z][1]["zeff"] for z in self.zs], dtype=np.float64)
         self.sp_arep = np.array([els[z][1]["arep"] for z in self.zs], dtype=np.float64)
         us_sp, us_l, us_val, us_level, us_kcn, us_shpoly, us_hard, us_ref = ([] for _ in range(8))
+        us_rec = []  # index of the unique shell in its element's parameter record
         us_alpha, us_coeff, us_nprim = [], [], []
         self.us_base = np.zeros(nsp, dtype=np.int32)
         self.nsh_of_sp = np.zeros(nsp, dtype=np.int32)
@@ -127,6 +128,7 @@ class SpeciesTables:
             self.nsh_of_sp[i] = len(shells)
             for l, a, c, k in shells:
                 us_sp.append(i)
+                us_rec.append(k)
                 us_l.append(l)
                 us_val.append(bool(e["valence"][k]))
                 us_level.append(e["levels"][k] * C.EV2AU)
@@ -143,6 +145,7 @@ class SpeciesTables:
                 us_nprim.append(len(a))
         self.nus = len(us_l)
         self.us_sp = np.array(us_sp, dtype=np.int32)
+        self.us_rec = np.array(us_rec, dtype=np.int32)
         self.us_l = np.array(us_l, dtype=np.int32)
         self.us_valence = np.array(us_val, dtype=bool)
         self.us_level = np.array(us_level)

@@ -28,6 +28,9 @@
  *                    (src/dxtb/_src/calculators/types/analytical.py:61-222)
  *                    with GFN1Hamiltonian.get_gradient
  *                    (src/dxtb/_src/xtb/gfn1.py:185-408)
+ *   xtb_param_grad   dE/d(Hamiltonian, Coulomb, repulsion parameters);
+ *                    replaces autograd with differentiable ParamModule
+ *                    entries (src/dxtb/_src/param/module/utils.py:123-240)
  *
  * Conventions
  *  - All array pointers are DEVICE pointers owned by the caller; the library
@@ -265,6 +268,19 @@ int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
                    const double* positions, const double* cn,
                    const double* gamma, const xtb_scf_state* state,
                    double* dedcn, double* grad, void* stream);
+
+/* Parameter gradient dE/dtheta of E = E_scf + (-TS) + E_rep at the converged
+ * density (the energy is stationary in it, so no response term); replaces
+ * autograd through the SCF with differentiable parameters
+ * (src/dxtb/_src/param/module/utils.py:123-240, examples/optim/param.py).
+ * Needs S and CN of the geometry and state.P / state.qsh of a finished
+ * xtb_scf.  Per molecule, 4*nus + 3*nsp doubles: dE/d level[nus],
+ * d kcn[nus], d shpoly[nus], d eta[nus] (ES2 shell hardness), d gam3[nsp],
+ * d zeff[nsp], d arep[nsp] (the xtb_params tables, atomic units).
+ * nus <= 32, nsp <= 16. */
+int32_t xtb_param_grad(const xtb_batch* batch, const xtb_params* par,
+                       const double* positions, const double* cn, const double* S,
+                       const xtb_scf_state* state, double* grad, void* stream);
 
 #ifdef __cplusplus
 }

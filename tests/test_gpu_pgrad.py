@@ -1,0 +1,90 @@
+"""Parameter gradients dE/dtheta against central finite differences of the
+GPU energies in the same parameters (h = 1e-5 x max(1, |theta|)): every
+level, kcn, shpoly and ES2 hardness of the unique shells and every gam3,
+zeff and arep of the species, on four molecules; and the calculator's
+mapping to the parameter file's names and units."""
+
+import numpy as np
+import pytest
+import torch
+
+from conftest import mol
+from dxtb_amd import constants as C
+from dxtb_amd import workloads as W
+from dxtb_amd.engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
+from dxtb_amd.param import SpeciesTables
+from helpers import TIGHT
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def _energies(mols, tb, dev, grad=False):
+    lay = BatchLayout([np.asarray(z) for z, _ in mols], tb, padded=False, chrg=np.zeros(len(mols)))
+    pipe = Pipeline(lay, DeviceTables(tb, dev), dev)
+    pos = torch.tensor(np.concatenate([p for _, p in mols]), dtype=torch.float64, device=dev)
+    pipe.run(pos, ScfOptions(**TIGHT), forces=False)
+    assert int(pipe.records()[:, 6].abs().max().item()) == 0
+    e = pipe.energies().cpu().numpy()
+    return (e, pipe.param_grad().cpu().numpy()) if grad else e
+
+
+def test_parameter_gradient_vs_finite_differences(fx):
+    rng = np.random.default_rng(15)  # the chain holds N and O
+    mols = [mol(fx, "h2o"), mol(fx, "ch4"), W.chain(rng, 8, p_n=0.3, p_o=0.3), W.water_cluster(rng, 4)]
+    dev = torch.device("cuda:0")
+    zs = np.unique(np.concatenate([z for z, _ in mols]))
+    tb = SpeciesTables(zs)
+    print(f"case: parameter gradients, species {list(tb.zs)}, {tb.nus} unique shells", flush=True)
+    assert list(tb.zs) == [1, 6, 7, 8]
+    _, g = _energies(mols, tb, dev, grad=True)
+    nus, nsp = tb.nus, len(tb.zs)
+    cols = [("us_level", u, u) for u in range(nus)] + [("us_kcn", u, nus + u) for u in range(nus)]
+    cols += [("us_shpoly", u, 2 * nus + u) for u in range(nus)] + [("us_hard", u, 3 * nus + u) for u in range(nus)]
+    cols += [("sp_gam3", i, 4 * nus + i) for i in range(nsp)] + [("sp_zeff", i, 4 * nus + nsp + i) for i in range(nsp)]
+    cols += [("sp_arep", i, 4 * nus + 2 * nsp + i) for i in range(nsp)]
+    worst = 0.0
+    for name, idx, col in cols:
+        base = getattr(tb, name)[idx]
+        h = 1e-5 * max(1.0, abs(base))
+        ep = []
+        for s in (1.0, -1.0):
+            t = SpeciesTables(zs)
+            getattr(t, name)[idx] = base + s * h
+            ep.append(_energies(mols, t, dev))
+        fd = (ep[0] - ep[1]) / (2 * h)
+        err = np.abs(fd - g[:, col]).max() / max(1.0, np.abs(fd).max())
+        worst = max(worst, err)
+        assert err < 1e-7, (name, idx, fd, g[:, col])
+    print(f"worst relative error {worst:.1e}", flush=True)
+
+
+def test_calculator_parameter_gradient_names_and_units(fx):
+    from dxtb_amd import GFN1Calculator
+    from dxtb_amd.param import elements_by_z
+
+    z, p = mol(fx, "h2o")
+    dev = torch.device("cuda:0")
+    calc = GFN1Calculator(torch.tensor(z, device=dev), dtype=torch.float64,
+                          opts={"x_atol": 1e-11, "x_atol_max": 1e-11, "maxiter": 300, "exclude": ["disp"]})
+    pos = torch.tensor(p, dtype=torch.float64, device=dev)
+    d = calc.get_param_gradient(pos)
+    assert set(d) == {"H", "O"}
+    tb = SpeciesTables(np.unique(z))
+    _, g = _energies([(z, p)], tb, dev, grad=True)
+    els = elements_by_z()
+    for i, zz in enumerate(tb.zs):
+        sym, e = els[zz]
+        us = np.nonzero(tb.us_sp == i)[0]
+        nus = tb.nus
+        lv = d[sym]["levels"].cpu().numpy()
+        assert lv.shape == (len(e["levels"]),)
+        assert np.allclose(lv[tb.us_rec[us]], g[0, us] * C.EV2AU, rtol=1e-12, atol=1e-14)
+        eta = g[0, 3 * nus + us]
+        assert abs(d[sym]["gam"].item() - float((eta * np.asarray(e["lgam"])[tb.us_rec[us]]).sum())) < 1e-12
+        assert abs(d[sym]["arep"].item() - g[0, 4 * nus + 2 * len(tb.zs) + i]) < 1e-14
