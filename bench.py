@@ -115,8 +115,17 @@ def forces_cost(reqs, device, opts, repeats: int = 3) -> dict:
             torch.cuda.synchronize()
             ts.append(time.perf_counter() - t0)
         out["forces" if forces else "energy"] = float(np.median(ts[1:]))
+    # parameter gradients of the same batch (after its last run)
+    ts = []
+    for k in range(repeats + 1):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        pipe.param_grad()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
     return {"energy_ms": round(out["energy"] * 1e3, 2), "energy_forces_ms": round(out["forces"] * 1e3, 2),
-            "ratio": round(out["forces"] / out["energy"], 3), "molecules": len(reqs)}
+            "ratio": round(out["forces"] / out["energy"], 3), "molecules": len(reqs),
+            "param_grad_ms": round(float(np.median(ts[1:])) * 1e3, 3)}
 
 
 def md_warm_start(device, opts, steps: int = 6) -> dict:
