@@ -115,6 +115,9 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_forces.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P, P]
     lib.xtb_param_grad.restype = i32
     lib.xtb_param_grad.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P]
+    lib.xtb_param_grad_basis.restype = i32
+    lib.xtb_param_grad_basis.argtypes = [bp, pp, P, P, P, P, ct.c_uint32, ct.c_double, P, P,
+                                         ct.POINTER(XtbScfState), P, i32, i32, P]
     if lib.xtb_abi_version() != 1:
         raise NativeLibraryError("libxtbhip.so ABI version mismatch")
     sizes = (i64 * 4)()
@@ -129,6 +132,7 @@ def load(path: str | os.PathLike | None = None):
 EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
+    "xtb_param_grad_basis",
 )
 
 

@@ -331,19 +331,20 @@ class GFN1Calculator:
         """dE/dtheta for the GFN1 parameters of the Hamiltonian, the Coulomb
         terms and the repulsion, by element and under the parameter file's
         names and units (src/dxtb/_src/param/gfn1/gfn1-xtb.toml): per shell
-        ``levels`` and ``kcn`` (eV), ``shpoly``, ``lgam``; per element
-        ``gam``, ``gam3``, ``zeff``, ``arep``.  Each value has a leading
+        ``levels`` and ``kcn`` (eV), ``shpoly``, ``lgam``, ``slater``; per
+        element ``gam``, ``gam3``, ``zeff``, ``arep``.  Each value has a leading
         batch dimension in batched mode.  Upstream obtains these by autograd
         with ParamModule.set_differentiable (param/module/utils.py:123-240);
         here the energy's stationarity in the density gives them from the
         converged density and charges without a response term."""
         from .param import elements_by_z
 
-        res = self._run(positions, chrg, spin, forces=False)
+        res = self._run(positions, chrg, spin, forces=True)
         if "large" in res:
             raise PropertyNotImplementedError("parameter gradients on the large-system path")
         g = res["pipeline"].param_grad()
         tb = self.tables
+        d_zeta = res["pipeline"].param_grad_basis(tb)
         nus, nsp = tb.nus, len(tb.zs)
         d_level, d_kcn, d_poly, d_eta = (g[:, k * nus:(k + 1) * nus] for k in range(4))
         d_gam3, d_zeff, d_arep = (g[:, 4 * nus + k * nsp:4 * nus + (k + 1) * nsp] for k in range(3))
@@ -366,7 +367,7 @@ class GFN1Calculator:
                 "levels": per_shell(d_level, C.EV2AU), "kcn": per_shell(d_kcn, C.EV2AU),
                 "shpoly": per_shell(d_poly), "lgam": per_shell(d_eta, float(e["gam"])),
                 "gam": (eta * lgam).sum(-1), "gam3": d_gam3[:, i], "zeff": d_zeff[:, i],
-                "arep": d_arep[:, i],
+                "arep": d_arep[:, i], "slater": per_shell(d_zeta),
             }
             out[sym] = d if self.batched else {k: v[0] for k, v in d.items()}
         return out

@@ -8,13 +8,14 @@
 // these derivatives by autograd through the SCF (examples/optim/param.py,
 // src/dxtb/_src/param/module/utils.py:123-240).
 //
-// Per molecule, rows of XTB_PG_STRIDE(nus, nsp) doubles:
+// xtb_param_grad writes per molecule a row of 4 nus + 3 nsp doubles:
 //   [0, nus)          dE/d level_s      (Hartree per Hartree)
 //   [nus, 2nus)       dE/d kcn_s
 //   [2nus, 3nus)      dE/d shpoly_s
 //   [3nus, 4nus)      dE/d eta_s        (ES2 shell hardness, lgam_s * gam_A)
 //   [4nus, 4nus+nsp)  dE/d gam3_A
 //   then dE/d zeff_A, dE/d arep_A.
+// xtb_param_grad_basis adds dE/dzeta of one Slater exponent to a column.
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -181,6 +182,91 @@ __global__ void __launch_bounds__(256) k_pgrad_atoms(xtb_batch b, xtb_params par
     }
 }
 
+// Basis exponents: dE/dzeta = sum over off-site shell-pair blocks of
+// [2 (P H~ - W) - P (v_mu + v_nu)] dS/dzeta, the expression of the analytic
+// forces with dS/dR replaced (H0 = S H~ with H~ free of zeta; on-site S is
+// the identity).  dS/dzeta by central differences of the contracted
+// overlaps with the shells' primitives rebuilt at zeta +- h on the host
+// (the tables (alpha, c)+-; `mask` marks the unique shells they change: the
+// shell itself and a shell orthogonalised against it).  One thread per
+// atom pair A < B of a 16 x 16 tile; result to column `col` of `out`.
+__global__ void __launch_bounds__(256) k_pgrad_basis(xtb_batch b, xtb_params par,
+                                                     const double* __restrict__ ap,
+                                                     const double* __restrict__ cp,
+                                                     const double* __restrict__ am,
+                                                     const double* __restrict__ cm, uint32_t mask,
+                                                     double inv2h, const double* __restrict__ pos,
+                                                     const double* __restrict__ cn, xtb_scf_state st,
+                                                     double* __restrict__ out, int col, int ncol) {
+    __shared__ double s_ap[PG_MAX_US * XTB_MAXPRIM], s_cp[PG_MAX_US * XTB_MAXPRIM];
+    __shared__ double s_am[PG_MAX_US * XTB_MAXPRIM], s_cm[PG_MAX_US * XTB_MAXPRIM];
+    __shared__ double red[4];
+    for (int x = threadIdx.x; x < par.nus * XTB_MAXPRIM; x += blockDim.x) {
+        s_ap[x] = ap[x];
+        s_cp[x] = cp[x];
+        s_am[x] = am[x];
+        s_cm[x] = cm[x];
+    }
+    __syncthreads();
+    const int* tl = b.tile_at + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int A = tl[1] + threadIdx.x / 16;
+    const int B = tl[2] + threadIdx.x % 16;
+    const int nat = b.mol_nat[m];
+    double acc = 0.0;
+    if (A < nat && B < nat && B > A) {
+        const int n = b.mol_n[m];
+        const int at0 = b.mol_atom0[m], sh0 = b.mol_sh0[m], vec0 = b.mol_vec0[m];
+        const size_t mat0 = (size_t)b.mol_mat0[m];
+        const double* P = st.P + mat0;
+        const double* W = st.T + mat0;
+        const double* v = st.pot + vec0;
+        const int ga = at0 + A, gb = at0 + B;
+        const double R[3] = {pos[3 * ga] - pos[3 * gb], pos[3 * ga + 1] - pos[3 * gb + 1],
+                             pos[3 * ga + 2] - pos[3 * gb + 2]};
+        const double dist = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        if (dist < par.int_cutoff && dist > 0.1) {
+            const double rab = par.sp_rad[b.atom_sp[ga]] + par.sp_rad[b.atom_sp[gb]];
+            const int sa0 = b.atom_sh0[ga], nsa = b.atom_nsh[ga];
+            const int sb0 = b.atom_sh0[gb], nsb = b.atom_nsh[gb];
+            for (int i = sa0; i < sa0 + nsa; ++i) {
+                const int usi = b.sh_us[sh0 + i];
+                const int li = par.us_l[usi], ni = 2 * li + 1, oi = b.sh_ao[sh0 + i];
+                const double hi = par.us_level[usi] - par.us_kcn[usi] * cn[ga];
+                for (int j = sb0; j < sb0 + nsb; ++j) {
+                    const int usj = b.sh_us[sh0 + j];
+                    if (!(((mask >> usi) | (mask >> usj)) & 1u)) continue;
+                    const int lj = par.us_l[usj], nj = 2 * lj + 1, oj = b.sh_ao[sh0 + j];
+                    const double hj = par.us_level[usj] - par.us_kcn[usj] * cn[gb];
+                    double Sp[3][3], Sm[3][3], dummy[3][3][3];
+                    shell_overlap<false>(li, lj, par.us_nprim[usi], s_ap + usi * XTB_MAXPRIM,
+                                         s_cp + usi * XTB_MAXPRIM, par.us_nprim[usj],
+                                         s_ap + usj * XTB_MAXPRIM, s_cp + usj * XTB_MAXPRIM, R, Sp, dummy);
+                    shell_overlap<false>(li, lj, par.us_nprim[usi], s_am + usi * XTB_MAXPRIM,
+                                         s_cm + usi * XTB_MAXPRIM, par.us_nprim[usj],
+                                         s_am + usj * XTB_MAXPRIM, s_cm + usj * XTB_MAXPRIM, R, Sm, dummy);
+                    double pi, dpi;
+                    dist_poly(dist, rab, par.us_shpoly[usi], par.us_shpoly[usj], pi, dpi);
+                    const double ht = 0.5 * (hi + hj) * par.us_K[usi * par.nus + usj] * pi;
+                    for (int x = 0; x < ni; ++x)
+                        for (int y = 0; y < nj; ++y) {
+                            const size_t e = (size_t)(oi + x) * n + (oj + y);
+                            const double sval = 2.0 * (P[e] * ht - W[e]) - P[e] * (v[oi + x] + v[oj + y]);
+                            acc += sval * (Sp[x][y] - Sm[x][y]) * inv2h;
+                        }
+                }
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double v = red[0] + red[1] + red[2] + red[3];
+        if (v != 0.0) atomicAdd(out + (size_t)m * ncol + col, v);
+    }
+}
+
 __global__ void k_pgrad_zero(double* __restrict__ a, int64_t n) {
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
         a[e] = 0.0;
@@ -205,5 +291,23 @@ extern "C" int32_t xtb_param_grad(const xtb_batch* batch, const xtb_params* par,
     if (batch->ntile_at > 0)
         hipLaunchKernelGGL(k_pgrad_atoms, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
                            positions, state->qsh, grad);
+    return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP;
+}
+
+extern "C" int32_t xtb_param_grad_basis(const xtb_batch* batch, const xtb_params* par,
+                                        const double* alpha_plus, const double* coeff_plus,
+                                        const double* alpha_minus, const double* coeff_minus,
+                                        uint32_t mask, double h, const double* positions,
+                                        const double* cn, const xtb_scf_state* state, double* grad,
+                                        int32_t col, int32_t ncol, void* stream) {
+    if (!batch || !par || !alpha_plus || !coeff_plus || !alpha_minus || !coeff_minus || !positions ||
+        !cn || !state || !grad || !(h > 0.0) || col < 0 || col >= ncol)
+        return XTB_EINVAL;
+    if (!state->P || !state->T || !state->pot) return XTB_EINVAL;
+    if (par->nus > PG_MAX_US || batch->nmol <= 0) return XTB_ETOOBIG;
+    if (batch->ntile_at > 0)
+        hipLaunchKernelGGL(k_pgrad_basis, dim3(batch->ntile_at), dim3(256), 0, (hipStream_t)stream, *batch,
+                           *par, alpha_plus, coeff_plus, alpha_minus, coeff_minus, mask, 0.5 / h,
+                           positions, cn, *state, grad, col, ncol);
     return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP;
 }

@@ -258,6 +258,7 @@ class Pipeline:
         self.qsh = z(int(L.sh0[-1]))
         self.out = torch.zeros(L.B * _lib.OUT_STRIDE, dtype=torch.float64, device=dev)
         self._kept = False  # the state buffers hold orbitals and potential to warm-start from
+        self._have_w = False  # the last run formed the energy-weighted density
         self.grad = z(3 * natt)
         self.dedcn = z(natt)
         self.hist_x = self.hist_f = None
@@ -322,6 +323,7 @@ class Pipeline:
             ev1.record(st)
             self.scf_events = (ev0, ev1)
         self._kept = bool(opts.warm_start)
+        self._have_w = bool(forces)
         if forces:
             self.run_forces(stream)
 
@@ -346,6 +348,47 @@ class Pipeline:
                                            self.cn.data_ptr(), self.S.data_ptr(), ct.byref(self.state),
                                            g.data_ptr(), ct.c_void_p(st.cuda_stream)),
                    "xtb_param_grad")
+        return g
+
+    def param_grad_basis(self, tables, rel_step: float = 1e-4, stream=None) -> torch.Tensor:
+        """dE/dzeta of the last run's energies for the Slater exponent of
+        every unique shell of `tables` (the SpeciesTables this pipeline's
+        DeviceTables were made from): (B, nus).  Needs a run with forces
+        (the energy-weighted density).  dS/dzeta is a central difference of
+        the contracted overlaps at zeta (1 +- rel_step)."""
+        from .param import SpeciesTables, elements_by_z
+
+        if not self._have_w:
+            raise RuntimeError("parameter gradients of the basis need a run with forces")
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        s = ct.c_void_p(st.cuda_stream)
+        els = elements_by_z()
+        nus = tables.nus
+        g = torch.zeros((self.L.B, nus), dtype=torch.float64, device=self.device)
+        keep = []
+        for u in range(nus):
+            z = tables.zs[int(tables.us_sp[u])]
+            zeta = list(els[z][1]["slater"])
+            k = int(tables.us_rec[u])
+            h = rel_step * zeta[k]
+            side = []
+            for sg in (1.0, -1.0):
+                zz = list(zeta)
+                zz[k] = zeta[k] + sg * h
+                t = SpeciesTables(tables.zs, slater={z: zz})
+                side.append((torch.as_tensor(t.us_alpha, device=self.device),
+                             torch.as_tensor(t.us_coeff, device=self.device), t))
+            tp, tm = side[0][2], side[1][2]
+            diff = (np.abs(tp.us_alpha - tm.us_alpha) + np.abs(tp.us_coeff - tm.us_coeff))
+            changed = diff.reshape(nus, -1).max(axis=1) > 0
+            mask = int(sum(1 << v for v in np.nonzero(changed)[0]))
+            keep.append(side)
+            _lib.check(self.lib.xtb_param_grad_basis(
+                ct.byref(self.batch), ct.byref(self.dt.s), side[0][0].data_ptr(), side[0][1].data_ptr(),
+                side[1][0].data_ptr(), side[1][1].data_ptr(), ct.c_uint32(mask), ct.c_double(h),
+                self.positions.data_ptr(), self.cn.data_ptr(), ct.byref(self.state), g.data_ptr(),
+                u, nus, s), "xtb_param_grad_basis")
+        st.synchronize()  # the perturbed tables in `keep` are read by the launches
         return g
 
     def scf_flops(self) -> float:

@@ -80,8 +80,9 @@ def orthogonalize(a1, c1, a2, c2):
 
 
 @lru_cache(maxsize=None)
-def species_shells(z: int):
-    """Contracted shells of element z: list of (l, alpha, coeff, record index)."""
+def species_shells(z: int, slater: tuple | None = None):
+    """Contracted shells of element z: list of (l, alpha, coeff, record index).
+    `slater` replaces the element's Slater exponents (one per shell)."""
     els = elements_by_z()
     if z not in els or z not in C.SUPPORTED_ELEMENTS:
         raise NotImplementedError(
@@ -89,9 +90,10 @@ def species_shells(z: int):
             f"Z in {C.SUPPORTED_ELEMENTS}."
         )
     _, e = els[z]
+    zeta = e["slater"] if slater is None else slater
     out = []
     for k, (n, l) in enumerate(e["shells"]):
-        a, c = slater_to_gauss(e["ngauss"][k], n, l, e["slater"][k])
+        a, c = slater_to_gauss(e["ngauss"][k], n, l, zeta[k])
         if not e["valence"][k]:
             # src/dxtb/_src/basis/bas.py:174-184
             pa, pc = out[k - 1][1], out[k - 1][2]
@@ -103,7 +105,8 @@ def species_shells(z: int):
 class SpeciesTables:
     """Flat parameter arrays for the species of a batch (host numpy)."""
 
-    def __init__(self, zs):
+    def __init__(self, zs, slater: dict | None = None):
+        """`slater` maps an atomic number to replacement Slater exponents."""
         par = load_gfn1()
         els = elements_by_z()
         self.zs = sorted(set(int(z) for z in zs))
@@ -124,7 +127,7 @@ class SpeciesTables:
         for i, z in enumerate(self.zs):
             _, e = els[z]
             self.us_base[i] = len(us_l)
-            shells = species_shells(z)
+            shells = species_shells(z, None if not slater or z not in slater else tuple(slater[z]))
             self.nsh_of_sp[i] = len(shells)
             for l, a, c, k in shells:
                 us_sp.append(i)

@@ -282,6 +282,21 @@ int32_t xtb_param_grad(const xtb_batch* batch, const xtb_params* par,
                        const double* positions, const double* cn, const double* S,
                        const xtb_scf_state* state, double* grad, void* stream);
 
+/* Parameter gradient for one Slater exponent: adds dE/dzeta to column `col`
+ * of grad [nmol x ncol] (caller zeroes it).  The shells' primitives rebuilt
+ * at zeta + h and zeta - h (alpha/coeff tables as in xtb_params, nus x 8);
+ * bit u of `mask` marks the unique shells they change.  Needs P, W (state.T
+ * of an xtb_scf with want_w) and the potential of the same batch.  dS/dzeta
+ * is the central difference of the contracted overlaps; the contraction with
+ * the energy's derivative in S is that of xtb_forces.  Replaces autograd
+ * with differentiable "slater" entries (examples/optim/param.py). */
+int32_t xtb_param_grad_basis(const xtb_batch* batch, const xtb_params* par,
+                             const double* alpha_plus, const double* coeff_plus,
+                             const double* alpha_minus, const double* coeff_minus,
+                             uint32_t mask, double h, const double* positions,
+                             const double* cn, const xtb_scf_state* state, double* grad,
+                             int32_t col, int32_t ncol, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

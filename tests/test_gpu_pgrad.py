@@ -1,8 +1,8 @@
 """Parameter gradients dE/dtheta against central finite differences of the
 GPU energies in the same parameters (h = 1e-5 x max(1, |theta|)): every
-level, kcn, shpoly and ES2 hardness of the unique shells and every gam3,
-zeff and arep of the species, on four molecules; and the calculator's
-mapping to the parameter file's names and units."""
+level, kcn, shpoly, ES2 hardness and Slater exponent of the unique shells
+and every gam3, zeff and arep of the species, on four molecules; and the
+calculator's mapping to the parameter file's names and units."""
 
 import numpy as np
 import pytest
@@ -28,10 +28,12 @@ def _energies(mols, tb, dev, grad=False):
     lay = BatchLayout([np.asarray(z) for z, _ in mols], tb, padded=False, chrg=np.zeros(len(mols)))
     pipe = Pipeline(lay, DeviceTables(tb, dev), dev)
     pos = torch.tensor(np.concatenate([p for _, p in mols]), dtype=torch.float64, device=dev)
-    pipe.run(pos, ScfOptions(**TIGHT), forces=False)
+    pipe.run(pos, ScfOptions(**TIGHT), forces=grad)
     assert int(pipe.records()[:, 6].abs().max().item()) == 0
     e = pipe.energies().cpu().numpy()
-    return (e, pipe.param_grad().cpu().numpy()) if grad else e
+    if not grad:
+        return e
+    return e, pipe.param_grad().cpu().numpy(), pipe.param_grad_basis(tb).cpu().numpy()
 
 
 def test_parameter_gradient_vs_finite_differences(fx):
@@ -42,7 +44,7 @@ def test_parameter_gradient_vs_finite_differences(fx):
     tb = SpeciesTables(zs)
     print(f"case: parameter gradients, species {list(tb.zs)}, {tb.nus} unique shells", flush=True)
     assert list(tb.zs) == [1, 6, 7, 8]
-    _, g = _energies(mols, tb, dev, grad=True)
+    _, g, gz = _energies(mols, tb, dev, grad=True)
     nus, nsp = tb.nus, len(tb.zs)
     cols = [("us_level", u, u) for u in range(nus)] + [("us_kcn", u, nus + u) for u in range(nus)]
     cols += [("us_shpoly", u, 2 * nus + u) for u in range(nus)] + [("us_hard", u, 3 * nus + u) for u in range(nus)]
@@ -62,6 +64,26 @@ def test_parameter_gradient_vs_finite_differences(fx):
         worst = max(worst, err)
         assert err < 1e-7, (name, idx, fd, g[:, col])
     print(f"worst relative error {worst:.1e}", flush=True)
+    # Slater exponents (the basis): central differences of the energy
+    from dxtb_amd.param import elements_by_z
+
+    els = elements_by_z()
+    worst = 0.0
+    for u in range(nus):
+        z = tb.zs[int(tb.us_sp[u])]
+        zeta = list(els[z][1]["slater"])
+        k = int(tb.us_rec[u])
+        h = 1e-5 * zeta[k]
+        ep = []
+        for s in (1.0, -1.0):
+            zz = list(zeta)
+            zz[k] += s * h
+            ep.append(_energies(mols, SpeciesTables(zs, slater={z: zz}), dev))
+        fd = (ep[0] - ep[1]) / (2 * h)
+        err = np.abs(fd - gz[:, u]).max() / max(1.0, np.abs(fd).max())
+        worst = max(worst, err)
+        assert err < 1e-6, (z, k, fd, gz[:, u])
+    print(f"slater: worst relative error {worst:.1e}", flush=True)
 
 
 def test_calculator_parameter_gradient_names_and_units(fx):
@@ -76,7 +98,7 @@ def test_calculator_parameter_gradient_names_and_units(fx):
     d = calc.get_param_gradient(pos)
     assert set(d) == {"H", "O"}
     tb = SpeciesTables(np.unique(z))
-    _, g = _energies([(z, p)], tb, dev, grad=True)
+    _, g, gz = _energies([(z, p)], tb, dev, grad=True)
     els = elements_by_z()
     for i, zz in enumerate(tb.zs):
         sym, e = els[zz]
@@ -88,3 +110,4 @@ def test_calculator_parameter_gradient_names_and_units(fx):
         eta = g[0, 3 * nus + us]
         assert abs(d[sym]["gam"].item() - float((eta * np.asarray(e["lgam"])[tb.us_rec[us]]).sum())) < 1e-12
         assert abs(d[sym]["arep"].item() - g[0, 4 * nus + 2 * len(tb.zs) + i]) < 1e-14
+        assert np.allclose(d[sym]["slater"].cpu().numpy()[tb.us_rec[us]], gz[0, us], rtol=1e-12, atol=1e-14)
