@@ -320,6 +320,7 @@ def main() -> int:
     # are held, the step ends when the oldest of them is done.  The timed
     # region ends when every chunk is done.
     torch.cuda.synchronize()
+    busy0 = server.stats.scf_busy_s
     t0 = time.perf_counter()
     served, held = [], []
     for k in range(a.steps):
@@ -337,6 +338,9 @@ def main() -> int:
     bad = sum(1 for r in served if server.results[r.rid].status != 0)
     value = a.steps * a.chunk / (t1 - t0)
     ms_step = (t1 - t0) * 1e3 / a.steps
+    # share of the CU-time the SCF kernel's workgroups held (one CU each)
+    ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+    occupancy = (server.stats.scf_busy_s - busy0) / (ncu * (t1 - t0))
     log(f"ragged: {value:.1f} molecules/s")
     # the same chunks one at a time, each drained before the next starts
     barrier = None
@@ -398,6 +402,7 @@ def main() -> int:
             "batching": f"continuous: step k hands its chunk to the serving loop, which holds "
                         f"at most {a.lookahead} chunks and admits each as streams free up",
             "streams": server.streams.__len__(), "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
+            "scf_cu_occupancy": round(occupancy, 3),
             "chunk_barrier_molecules_per_s": None if barrier is None else round(barrier, 3),
             "padded_dropin_molecules_per_s": None if padded is None else round(padded, 3),
             "ragged_over_padded": None if padded is None else round(value / padded, 3),

@@ -332,7 +332,9 @@ class GFN1Calculator:
         terms and the repulsion, by element and under the parameter file's
         names and units (src/dxtb/_src/param/gfn1/gfn1-xtb.toml): per shell
         ``levels`` and ``kcn`` (eV), ``shpoly``, ``lgam``, ``slater``; per
-        element ``gam``, ``gam3``, ``zeff``, ``arep``.  Each value has a leading
+        element ``gam``, ``gam3``, ``zeff``, ``arep``, ``en``; and under
+        ``"hamiltonian"`` the pair scaling's ``shell`` (ss, sp, pp),
+        ``kpol``, ``enscale`` and ``kpair`` entries.  Each value has a leading
         batch dimension in batched mode.  Upstream obtains these by autograd
         with ParamModule.set_differentiable (param/module/utils.py:123-240);
         here the energy's stationarity in the density gives them from the
@@ -346,6 +348,23 @@ class GFN1Calculator:
         tb = self.tables
         d_zeta = res["pipeline"].param_grad_basis(tb)
         nus, nsp = tb.nus, len(tb.zs)
+        # the pair scaling K_ab: chain rule to the Hamiltonian section's
+        # parameters and the electronegativities
+        from .param import load_gfn1, shell_pair_k
+
+        d_K = g[:, 4 * nus + 3 * nsp:].reshape(-1, nus, nus)
+        en = {z: e["en"] for z, (_, e) in elements_by_z().items()}
+        _, dks = shell_pair_k(tb, load_gfn1()["hamiltonian"], en, grad=True)
+        ham: dict = {"shell": {}, "kpair": {}}
+        d_en = {}
+        for name, dk in dks:
+            v = (d_K * torch.as_tensor(dk, device=g.device)).sum((-1, -2))
+            if name[0] in ("shell", "kpair"):
+                ham[name[0]][name[1]] = v if self.batched else v[0]
+            elif name[0] == "en":
+                d_en[name[1]] = v
+            else:
+                ham[name[0]] = v if self.batched else v[0]
         d_level, d_kcn, d_poly, d_eta = (g[:, k * nus:(k + 1) * nus] for k in range(4))
         d_gam3, d_zeff, d_arep = (g[:, 4 * nus + k * nsp:4 * nus + (k + 1) * nsp] for k in range(3))
         els = elements_by_z()
@@ -368,8 +387,10 @@ class GFN1Calculator:
                 "shpoly": per_shell(d_poly), "lgam": per_shell(d_eta, float(e["gam"])),
                 "gam": (eta * lgam).sum(-1), "gam3": d_gam3[:, i], "zeff": d_zeff[:, i],
                 "arep": d_arep[:, i], "slater": per_shell(d_zeta),
+                "en": d_en.get(z, torch.zeros_like(d_arep[:, i])),
             }
             out[sym] = d if self.batched else {k: v[0] for k, v in d.items()}
+        out["hamiltonian"] = ham
         return out
 
     def get_iterations(self, positions, chrg=0, spin=None):

@@ -14,7 +14,8 @@
 //   [2nus, 3nus)      dE/d shpoly_s
 //   [3nus, 4nus)      dE/d eta_s        (ES2 shell hardness, lgam_s * gam_A)
 //   [4nus, 4nus+nsp)  dE/d gam3_A
-//   then dE/d zeff_A, dE/d arep_A.
+//   then dE/d zeff_A, dE/d arep_A, and dE/d K[a][b] (nus x nus, the H0
+//   pair scaling K_ab as each off-site shell pair uses it).
 // xtb_param_grad_basis adds dE/dzeta of one Slater exponent to a column.
 #include <hip/hip_runtime.h>
 
@@ -26,8 +27,10 @@ namespace xtb {
 constexpr int PG_MAX_US = 32;
 constexpr int PG_MAX_SP = 16;
 
+__device__ __host__ inline int pg_stride(int nus, int nsp) { return 4 * nus + 3 * nsp + nus * nus; }
+
 __device__ inline double* pg_row(double* out, int m, int nus, int nsp) {
-    return out + (size_t)m * (4 * nus + 3 * nsp);
+    return out + (size_t)m * pg_stride(nus, nsp);
 }
 
 // One thread per shell pair (i <= j) of a 16 x 16 tile of one molecule:
@@ -45,8 +48,10 @@ __global__ void __launch_bounds__(256) k_pgrad_shells(xtb_batch b, xtb_params pa
                                                       const double* __restrict__ qsh,
                                                       double* __restrict__ out) {
     __shared__ double acc[4 * PG_MAX_US];
+    __shared__ double acck[PG_MAX_US * PG_MAX_US];
     const int nus = par.nus;
     for (int x = threadIdx.x; x < 4 * PG_MAX_US; x += blockDim.x) acc[x] = 0.0;
+    for (int x = threadIdx.x; x < nus * nus; x += blockDim.x) acck[x] = 0.0;
     __syncthreads();
     const int* tl = b.tile_sh + 3 * blockIdx.x;
     const int m = tl[0];
@@ -98,6 +103,7 @@ __global__ void __launch_bounds__(256) k_pgrad_shells(xtb_batch b, xtb_params pa
                 const double gh = g * (hi + hj);
                 atomicAdd(&acc[2 * PG_MAX_US + usi], gh * rr * (1.0 + kj * rr));
                 atomicAdd(&acc[2 * PG_MAX_US + usj], gh * rr * (1.0 + ki * rr));
+                atomicAdd(&acck[usi * nus + usj], op * (hi + hj) * pi);
             }
         }
         // ES2 (k_gamma's definitions)
@@ -128,6 +134,9 @@ __global__ void __launch_bounds__(256) k_pgrad_shells(xtb_batch b, xtb_params pa
         const double v = acc[(x / nus) * PG_MAX_US + x % nus];
         if (v != 0.0) atomicAdd(row + x, v);
     }
+    double* rowk = row + 4 * nus + 3 * par.nsp;
+    for (int x = threadIdx.x; x < nus * nus; x += blockDim.x)
+        if (acck[x] != 0.0) atomicAdd(rowk + x, acck[x]);
 }
 
 // One thread per atom pair (A <= B) of a 16 x 16 tile of one molecule:
@@ -283,7 +292,7 @@ extern "C" int32_t xtb_param_grad(const xtb_batch* batch, const xtb_params* par,
     if (!state->P || !state->qsh) return XTB_EINVAL;
     if (par->nus > PG_MAX_US || par->nsp > PG_MAX_SP || batch->nmol <= 0) return XTB_ETOOBIG;
     hipStream_t s = (hipStream_t)stream;
-    const int64_t tot = (int64_t)batch->nmol * (4 * par->nus + 3 * par->nsp);
+    const int64_t tot = (int64_t)batch->nmol * pg_stride(par->nus, par->nsp);
     hipLaunchKernelGGL(k_pgrad_zero, dim3(64), dim3(256), 0, s, grad, tot);
     if (batch->ntile_sh > 0)
         hipLaunchKernelGGL(k_pgrad_shells, dim3(batch->ntile_sh), dim3(256), 0, s, *batch, *par,

@@ -338,12 +338,13 @@ class Pipeline:
 
     def param_grad(self, stream=None) -> torch.Tensor:
         """dE/dtheta of the last run's energies for the species tables'
-        parameters: (B, 4 nus + 3 nsp) = level, kcn, shpoly, eta (per unique
-        shell), gam3, zeff, arep (per species), atomic units."""
+        parameters: (B, 4 nus + 3 nsp + nus^2) = level, kcn, shpoly, eta (per
+        unique shell), gam3, zeff, arep (per species), us_K (per ordered
+        unique-shell pair), atomic units."""
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         tb = self.dt
         nus, nsp = int(tb.s.nus), int(tb.s.nsp)
-        g = torch.empty((self.L.B, 4 * nus + 3 * nsp), dtype=torch.float64, device=self.device)
+        g = torch.empty((self.L.B, 4 * nus + 3 * nsp + nus * nus), dtype=torch.float64, device=self.device)
         _lib.check(self.lib.xtb_param_grad(ct.byref(self.batch), ct.byref(tb.s), self.positions.data_ptr(),
                                            self.cn.data_ptr(), self.S.data_ptr(), ct.byref(self.state),
                                            g.data_ptr(), ct.c_void_p(st.cuda_stream)),

@@ -163,27 +163,62 @@ class SpeciesTables:
 
     def _shell_pair_k(self, par) -> np.ndarray:
         """K_ab of src/dxtb/_src/xtb/gfn1.py:66-165 and xtb/base.py:316-330."""
-        h = par["hamiltonian"]
-        lab = "spdf"
-        shell = h["shell"]
-        kpair = h["kpair"]
-        els = elements_by_z()
-        K = np.zeros((self.nus, self.nus))
-        for i in range(self.nus):
-            for j in range(self.nus):
-                li, lj = int(self.us_l[i]), int(self.us_l[j])
-                vi, vj = bool(self.us_valence[i]), bool(self.us_valence[j])
-                kii = shell.get(lab[li] * 2, 1.0) if vi else h["kpol"]
-                kjj = shell.get(lab[lj] * 2, 1.0) if vj else h["kpol"]
-                if vi and vj:
-                    hs = shell.get(lab[li] + lab[lj], shell.get(lab[lj] + lab[li], 0.5 * (kii + kjj)))
-                    za = self.zs[self.us_sp[i]]
-                    zb = self.zs[self.us_sp[j]]
-                    sa, ea = els[za]
-                    sb, eb = els[zb]
-                    kp = kpair.get(f"{sa}-{sb}", kpair.get(f"{sb}-{sa}", 1.0))
-                    x = 1.0 + h["enscale"] * (ea["en"] - eb["en"]) ** 2
-                    K[i, j] = hs * kp * x
+        return shell_pair_k(self, par["hamiltonian"], {z: e["en"] for z, (_, e) in elements_by_z().items()})
+
+
+def shell_pair_k(tb, h: dict, en: dict, grad: bool = False):
+    """K_ab = hscale(l_a, l_b) kpair(A, B) (1 + enscale (EN_A - EN_B)^2) for
+    two valence shells, (k_a + k_b)/2 otherwise (k = shell scaling, or kpol
+    for a polarisation shell), over the unique shells of `tb`, from the
+    Hamiltonian section `h` and electronegativities `en` (by atomic number).
+    With `grad`, also the derivatives: a list of (name, dK) with dK (nus, nus)
+    for every parameter K depends on (("shell", key), ("kpol",),
+    ("enscale",), ("kpair", key), ("en", z))."""
+    lab = "spdf"
+    shell = h["shell"]
+    kpair = h["kpair"]
+    els = elements_by_z()
+    nus = tb.nus
+    K = np.zeros((nus, nus))
+    dK: dict = {}
+
+    def add(name, i, j, v):
+        if grad:
+            dK.setdefault(name, np.zeros((nus, nus)))[i, j] += v
+
+    for i in range(nus):
+        for j in range(nus):
+            li, lj = int(tb.us_l[i]), int(tb.us_l[j])
+            vi, vj = bool(tb.us_valence[i]), bool(tb.us_valence[j])
+            kii = shell.get(lab[li] * 2, 1.0) if vi else h["kpol"]
+            kjj = shell.get(lab[lj] * 2, 1.0) if vj else h["kpol"]
+            if vi and vj:
+                za, zb = tb.zs[tb.us_sp[i]], tb.zs[tb.us_sp[j]]
+                sa, sb = els[za][0], els[zb][0]
+                kkey = f"{sa}-{sb}" if f"{sa}-{sb}" in kpair else (f"{sb}-{sa}" if f"{sb}-{sa}" in kpair else None)
+                kp = kpair[kkey] if kkey else 1.0
+                de = en[za] - en[zb]
+                x = 1.0 + h["enscale"] * de * de
+                hkey = lab[li] + lab[lj] if lab[li] + lab[lj] in shell else (
+                    lab[lj] + lab[li] if lab[lj] + lab[li] in shell else None)
+                hs = shell[hkey] if hkey else 0.5 * (kii + kjj)
+                K[i, j] = hs * kp * x
+                if hkey:
+                    add(("shell", hkey), i, j, kp * x)
                 else:
-                    K[i, j] = 0.5 * (kii + kjj)
-        return K
+                    for key in (lab[li] * 2, lab[lj] * 2):
+                        if key in shell:
+                            add(("shell", key), i, j, 0.5 * kp * x)
+                if kkey:
+                    add(("kpair", kkey), i, j, hs * x)
+                add(("enscale",), i, j, hs * kp * de * de)
+                add(("en", za), i, j, hs * kp * h["enscale"] * 2.0 * de)
+                add(("en", zb), i, j, -hs * kp * h["enscale"] * 2.0 * de)
+            else:
+                K[i, j] = 0.5 * (kii + kjj)
+                for v, l in ((vi, li), (vj, lj)):
+                    if not v:
+                        add(("kpol",), i, j, 0.5)
+                    elif lab[l] * 2 in shell:
+                        add(("shell", lab[l] * 2), i, j, 0.5)
+    return (K, list(dK.items())) if grad else K

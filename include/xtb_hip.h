@@ -274,9 +274,10 @@ int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
  * autograd through the SCF with differentiable parameters
  * (src/dxtb/_src/param/module/utils.py:123-240, examples/optim/param.py).
  * Needs S and CN of the geometry and state.P / state.qsh of a finished
- * xtb_scf.  Per molecule, 4*nus + 3*nsp doubles: dE/d level[nus],
+ * xtb_scf.  Per molecule, 4*nus + 3*nsp + nus*nus doubles: dE/d level[nus],
  * d kcn[nus], d shpoly[nus], d eta[nus] (ES2 shell hardness), d gam3[nsp],
- * d zeff[nsp], d arep[nsp] (the xtb_params tables, atomic units).
+ * d zeff[nsp], d arep[nsp], d us_K[nus][nus] (the xtb_params tables,
+ * atomic units; us_K as each off-site shell pair reads it).
  * nus <= 32, nsp <= 16. */
 int32_t xtb_param_grad(const xtb_batch* batch, const xtb_params* par,
                        const double* positions, const double* cn, const double* S,

@@ -84,6 +84,22 @@ def test_parameter_gradient_vs_finite_differences(fx):
         worst = max(worst, err)
         assert err < 1e-6, (z, k, fd, gz[:, u])
     print(f"slater: worst relative error {worst:.1e}", flush=True)
+    # the pair scaling K_ab as each shell pair reads it (ordered entries)
+    worst = 0.0
+    kcols = 4 * nus + 3 * nsp
+    for a_ in range(nus):
+        for b_ in range(nus):
+            h = 1e-5 * max(1.0, abs(tb.us_K[a_, b_]))
+            ep = []
+            for s in (1.0, -1.0):
+                t = SpeciesTables(zs)
+                t.us_K[a_, b_] += s * h
+                ep.append(_energies(mols, t, dev))
+            fd = (ep[0] - ep[1]) / (2 * h)
+            err = np.abs(fd - g[:, kcols + a_ * nus + b_]).max() / max(1.0, np.abs(fd).max())
+            worst = max(worst, err)
+            assert err < 1e-7, (a_, b_, fd, g[:, kcols + a_ * nus + b_])
+    print(f"K: worst relative error {worst:.1e}", flush=True)
 
 
 def test_calculator_parameter_gradient_names_and_units(fx):
@@ -96,7 +112,7 @@ def test_calculator_parameter_gradient_names_and_units(fx):
                           opts={"x_atol": 1e-11, "x_atol_max": 1e-11, "maxiter": 300, "exclude": ["disp"]})
     pos = torch.tensor(p, dtype=torch.float64, device=dev)
     d = calc.get_param_gradient(pos)
-    assert set(d) == {"H", "O"}
+    assert set(d) == {"H", "O", "hamiltonian"}
     tb = SpeciesTables(np.unique(z))
     _, g, gz = _energies([(z, p)], tb, dev, grad=True)
     els = elements_by_z()
@@ -111,3 +127,34 @@ def test_calculator_parameter_gradient_names_and_units(fx):
         assert abs(d[sym]["gam"].item() - float((eta * np.asarray(e["lgam"])[tb.us_rec[us]]).sum())) < 1e-12
         assert abs(d[sym]["arep"].item() - g[0, 4 * nus + 2 * len(tb.zs) + i]) < 1e-14
         assert np.allclose(d[sym]["slater"].cpu().numpy()[tb.us_rec[us]], gz[0, us], rtol=1e-12, atol=1e-14)
+    # Hamiltonian-section parameters: central differences through K
+    import copy
+
+    from dxtb_amd.param import load_gfn1, shell_pair_k
+
+    ham = load_gfn1()["hamiltonian"]
+    en = {zz: e["en"] for zz, (_, e) in elements_by_z().items()}
+    checks = [("shell", "ss"), ("shell", "sp"), ("shell", "pp"), ("kpol",), ("enscale",), ("kpair", "H-H"),
+              ("en", 8)]
+    for name in checks:
+        vals = []
+        for s in (1.0, -1.0):
+            hh, ee = copy.deepcopy(ham), dict(en)
+            h = 1e-5
+            if name[0] in ("shell", "kpair"):
+                hh[name[0]][name[1]] += s * h
+            elif name[0] == "en":
+                ee[name[1]] += s * h
+            else:
+                hh[name[0]] += s * h
+            t = SpeciesTables(np.unique(z))
+            t.us_K = shell_pair_k(t, hh, ee)
+            vals.append(_energies([(z, p)], t, dev)[0])
+        fd = (vals[0] - vals[1]) / 2e-5
+        if name[0] in ("shell", "kpair"):
+            an = d["hamiltonian"][name[0]].get(name[1], torch.zeros(())).item()
+        elif name[0] == "en":
+            an = d["O"]["en"].item()
+        else:
+            an = d["hamiltonian"][name[0]].item()
+        assert abs(fd - an) < 1e-7 * max(1.0, abs(fd)), (name, fd, an)
