@@ -236,10 +236,10 @@ def measured_traffic(chunk: int, warmup: int) -> float | None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--lookahead", type=int, default=4,
                     help="chunks the serving loop holds (waiting or in flight) at most")
-    ap.add_argument("--hw-queues", type=int, default=8,
+    ap.add_argument("--hw-queues", type=int, default=12,
                     help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-32)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=600)

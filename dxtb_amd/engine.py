@@ -18,6 +18,7 @@ optionally the analytic gradient.
 from __future__ import annotations
 
 import ctypes as ct
+import functools
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -49,22 +50,23 @@ def alpha_beta(nel: np.ndarray, spin) -> tuple[np.ndarray, np.ndarray]:
     return np.round(na), np.round(nb)
 
 
+@functools.lru_cache(maxsize=None)
+def _triu16(t: int) -> np.ndarray:
+    """(i0, j0) of the upper-triangle 16x16 tiles of a t x t tile grid."""
+    iu, ju = np.triu_indices(t)
+    return np.stack([iu * 16, ju * 16], axis=1).astype(np.int32)
+
+
 def _tiles(counts: np.ndarray) -> np.ndarray:
     """Upper-triangle 16x16 tiles of every molecule's pair space: rows (mol, i0, j0)."""
-    out = []
-    for m, c in enumerate(counts):
-        t = (int(c) + 15) // 16
-        if t == 0:
-            continue
-        iu, ju = np.triu_indices(t)
-        blk = np.empty((len(iu), 3), dtype=np.int32)
-        blk[:, 0] = m
-        blk[:, 1] = iu * 16
-        blk[:, 2] = ju * 16
-        out.append(blk)
-    if not out:
+    t = (np.asarray(counts, dtype=np.int64) + 15) // 16
+    ntile = t * (t + 1) // 2
+    if ntile.sum() == 0:
         return np.zeros((0, 3), dtype=np.int32)
-    return np.concatenate(out)
+    out = np.empty((int(ntile.sum()), 3), dtype=np.int32)
+    out[:, 0] = np.repeat(np.arange(len(t), dtype=np.int32), ntile)
+    out[:, 1:] = np.concatenate([_triu16(int(x)) for x in t if x > 0])
+    return out
 
 
 @dataclass
@@ -237,14 +239,29 @@ class Pipeline:
         self.lib = _lib.load()
         dev = self.device
         self.d = {}
-        # pinned staging + asynchronous copies: a pageable copy would block
-        # the host until the stream (or a busy stream sharing its hardware
-        # queue) reaches it.  The staging tensors live as long as the pipeline.
-        self._staging = []
-        for k, a in layout.arrays.items():
-            h = torch.from_numpy(np.ascontiguousarray(a)).pin_memory()
-            self._staging.append(h)
-            self.d[k] = h.to(dev, non_blocking=True)
+        # every layout array (and the launch order) packed into one pinned
+        # staging buffer and one asynchronous copy: a pageable copy would
+        # block the host until the stream (or a busy stream sharing its
+        # hardware queue) reaches it, and one copy costs less host time than
+        # twenty.  The staging buffer lives as long as the pipeline.
+        order = np.argsort(-layout.n, kind="stable").astype(np.int32)  # largest first
+        arrays = dict(layout.arrays)
+        arrays["order"] = order
+        offs, total = {}, 0
+        for k, a in arrays.items():
+            a = np.ascontiguousarray(a)
+            arrays[k] = a
+            offs[k] = total
+            total += (a.nbytes + 255) // 256 * 256
+        host = np.zeros(max(total, 256), dtype=np.uint8)
+        for k, a in arrays.items():
+            host[offs[k]:offs[k] + a.nbytes] = a.view(np.uint8).reshape(-1)
+        self._staging = torch.from_numpy(host).pin_memory()
+        buf = self._staging.to(dev, non_blocking=True)
+        tdt = {np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
+               np.dtype(np.float64): torch.float64}
+        for k, a in arrays.items():
+            self.d[k] = buf[offs[k]:offs[k] + max(a.nbytes, a.itemsize)].view(tdt[a.dtype])[: a.size]
         z = lambda k: torch.empty(k, dtype=torch.float64, device=dev)
         L = layout
         nm, nv = L.nmat_total, L.nvec_total
@@ -262,8 +279,7 @@ class Pipeline:
         self.grad = z(3 * natt)
         self.dedcn = z(natt)
         self.hist_x = self.hist_f = None
-        order = np.argsort(-L.n, kind="stable").astype(np.int32)  # largest first
-        self.order = torch.as_tensor(order, device=dev)
+        self.order = self.d["order"]
         a = self.d
         self.batch = _lib.XtbBatch(
             nmol=L.B, nat_total=natt, nsh_total=int(L.sh0[-1]), nvec_total=nv,
