@@ -123,7 +123,7 @@ __device__ inline v4d v4zero() {
 #endif
 
 // With `upper` (M == N, a product known to be symmetric) only the tiles on
-// and above the diagonal are formed; wg_mirror_lower fills the rest.
+// and above the diagonal are formed; gemm_sym mirrors the rest.
 // With EPI, C = alpha op(A) op(B) + s1 E1 + s2 E2 (E1, E2 with C's leading
 // dimension; beta unused): an elementwise update fused into the epilogue.
 template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false>

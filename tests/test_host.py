@@ -133,3 +133,49 @@ def test_warm_start_flags():
     assert (s.keep_state, s.warm_start) == (1, 1)
     s = ScfOptions().to_struct(want_w=False, padded=False, warm=True)
     assert (s.keep_state, s.warm_start) == (0, 0)
+
+
+def test_tile_lists_cover_each_pair_space_once():
+    """The 16 x 16 tiles of every molecule's upper pair triangle, vectorised
+    over the batch: each (i0 <= j0) block exactly once, in molecule order."""
+    from dxtb_amd.engine import _tiles
+
+    counts = np.array([0, 5, 17, 33, 1, 0, 100])
+    t = _tiles(counts)
+    want = []
+    for m, c in enumerate(counts):
+        k = (c + 15) // 16
+        want += [(m, 16 * i, 16 * j) for i in range(k) for j in range(i, k)]
+    assert [tuple(r) for r in t] == want
+    assert _tiles(np.array([0, 0])).shape == (0, 3)
+
+
+def test_pair_scaling_derivatives_match_finite_differences():
+    """param.shell_pair_k returns K_ab and its derivatives in the Hamiltonian
+    section's parameters and the electronegativities (the chain rule behind
+    the calculator's parameter gradients); central differences agree."""
+    import copy
+
+    from dxtb_amd.param import SpeciesTables, elements_by_z, load_gfn1, shell_pair_k
+
+    tb = SpeciesTables([1, 6, 7, 8])
+    ham = load_gfn1()["hamiltonian"]
+    en = {z: e["en"] for z, (_, e) in elements_by_z().items()}
+    K, dks = shell_pair_k(tb, ham, en, grad=True)
+    assert np.array_equal(K, tb.us_K)
+    names = {n for n, _ in dks}
+    assert {("shell", "ss"), ("shell", "sp"), ("shell", "pp"), ("kpol",), ("enscale",), ("kpair", "H-H"),
+            ("en", 1), ("en", 8)} <= names
+    h = 1e-6
+    for name, dk in dks:
+        ks = []
+        for s in (1.0, -1.0):
+            hh, ee = copy.deepcopy(ham), dict(en)
+            if name[0] in ("shell", "kpair"):
+                hh[name[0]][name[1]] += s * h
+            elif name[0] == "en":
+                ee[name[1]] += s * h
+            else:
+                hh[name[0]] += s * h
+            ks.append(shell_pair_k(tb, hh, ee))
+        assert np.abs((ks[0] - ks[1]) / (2 * h) - dk).max() < 1e-8, name
