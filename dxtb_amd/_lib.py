@@ -67,7 +67,7 @@ class XtbScfOpts(ct.Structure):
         ("x_atol", f64), ("x_atol_max", f64), ("kt", f64), ("damp", f64), ("damp_init", f64),
         ("diag_offset", f64), ("fermi_thresh", f64), ("jacobi_tol", f64), ("eig_inexact", f64),
         ("bd_gap_kt", f64), ("pd_kmax", f64), ("first_tol", f64), ("ortho", i32), ("orbitals", i32),
-        ("keep_state", i32), ("warm_start", i32),
+        ("keep_state", i32), ("warm_start", i32), ("soft_start", i32),
     ]
 
 

@@ -55,7 +55,7 @@ def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
     for key, attr in (
         ("maxiter", "maxiter"), ("mix_guess", "mix_guess"), ("damp", "damp"),
         ("damp_init", "damp_init"), ("damp_generations", "generations"),
-        ("damp_diagonal_offset", "diag_offset"), ("x_atol", "x_atol"),
+        ("damp_diagonal_offset", "diag_offset"), ("damp_soft_start", "soft_start"), ("x_atol", "x_atol"),
         ("x_atol_max", "x_atol_max"), ("fermi_etemp", "fermi_etemp"),
         ("fermi_maxiter", "fermi_maxiter"), ("fermi_thresh", "fermi_thresh"),
         # not an upstream option: successive calls on the same molecules
@@ -67,12 +67,10 @@ def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
     scp = str(rest.pop("scp_mode", "potential")).lower()
     if scp not in ("potential", "1"):
         raise NotImplementedError(f"scp_mode '{scp}': only 'potential' is implemented")
-    if rest.pop("damp_soft_start", True) is not True:
-        raise NotImplementedError("damp_soft_start=False is not implemented")
     if rest.pop("damp_dynamic", False):
         raise NotImplementedError("damp_dynamic is not implemented")
-    if not 1 <= int(o.generations) <= 5:
-        raise ValueError("damp_generations must be in 1..5")
+    if not 1 <= int(o.generations) <= 8:
+        raise ValueError("damp_generations must be in 1..8")
     return o, rest
 
 

@@ -813,6 +813,8 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
 // a ring buffer of G = generations+1 slots.  Returns the convergence flag of
 // the step (mixer/base.py:229-256 on the step's residual).
 // ---------------------------------------------------------------------------
+constexpr int MIX_MAXG = 8;  // largest history (generations) the mixer takes
+
 struct Mixer {
     int step, head, G;
     bool soft;  // upstream's soft start; a warm start, already near the fixed point, skips it
@@ -844,65 +846,71 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
     linf = wg_max(linf, sh.red);
     const bool conv = (sqrt(l2) < o.x_atol) && (linf < o.x_atol_max);
     *resid_inf = linf;
-    double theta[5] = {0, 0, 0, 0, 0};
+    double theta[MIX_MAXG];
+#pragma unroll
+    for (int h = 0; h < MIX_MAXG; ++h) theta[h] = 0.0;
     int nn = 0;
     if (mx.step > gen || !mx.soft) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
     if (nn > 0) {
-        // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn (nn <= 5)
-        double acc[20];
+        // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn
+        constexpr int NA = MIX_MAXG * (MIX_MAXG + 1) / 2, NACC = NA + MIX_MAXG;
+        double acc[NACC];
 #pragma unroll
-        for (int k = 0; k < 20; ++k) acc[k] = 0.0;
+        for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
         for (int i = threadIdx.x; i < n; i += NT) {
-            double d[5];
+            double d[MIX_MAXG];
             const double fi = f0[i];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) d[h] = (h < nn) ? fi - Fh(h + 1)[i] : 0.0;
+            for (int h = 0; h < MIX_MAXG; ++h) d[h] = (h < nn) ? fi - Fh(h + 1)[i] : 0.0;
             int k = 0;
 #pragma unroll
-            for (int h = 0; h < 5; ++h)
+            for (int h = 0; h < MIX_MAXG; ++h)
 #pragma unroll
-                for (int j = h; j < 5; ++j) acc[k++] += d[h] * d[j];
+                for (int j = h; j < MIX_MAXG; ++j) acc[k++] += d[h] * d[j];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) acc[15 + h] += d[h] * fi;
+            for (int h = 0; h < MIX_MAXG; ++h) acc[NA + h] += d[h] * fi;
         }
-        wg_sum_n<20>(acc, sh.red);
+        // the reduction scratch holds 256 doubles: 32 sums (x 8 waves) per pass
+        wg_sum_part<NACC, 0, 32>(acc, sh.red);
+        wg_sum_part<NACC, 32, NACC - 32>(acc, sh.red);
         // solve (a * diag(1 + off^2)) theta = b by Gaussian elimination with
         // partial pivoting; one thread, system held in LDS
-        ldsf* a = sh.red;          // [5][6]
-        ldsf* th = sh.red + 32;    // [5]
+        constexpr int W = MIX_MAXG + 1;
+        ldsf* a = sh.red;                  // [MIX_MAXG][W]
+        ldsf* th = sh.red + MIX_MAXG * W;  // [MIX_MAXG]
         if (threadIdx.x == 0) {
             int k = 0;
 #pragma unroll
-            for (int h = 0; h < 5; ++h)
+            for (int h = 0; h < MIX_MAXG; ++h)
 #pragma unroll
-                for (int j = h; j < 5; ++j) { a[h * 6 + j] = acc[k]; a[j * 6 + h] = acc[k]; ++k; }
+                for (int j = h; j < MIX_MAXG; ++j) { a[h * W + j] = acc[k]; a[j * W + h] = acc[k]; ++k; }
 #pragma unroll
-            for (int h = 0; h < 5; ++h) a[h * 6 + 5] = acc[15 + h];
-            for (int h = 0; h < nn; ++h) a[h * 6 + h] *= 1.0 + o.diag_offset * o.diag_offset;
+            for (int h = 0; h < MIX_MAXG; ++h) a[h * W + MIX_MAXG] = acc[NA + h];
+            for (int h = 0; h < nn; ++h) a[h * W + h] *= 1.0 + o.diag_offset * o.diag_offset;
             for (int c = 0; c < nn; ++c) {
                 int piv = c;
                 for (int r = c + 1; r < nn; ++r)
-                    if (fabs(a[r * 6 + c]) > fabs(a[piv * 6 + c])) piv = r;
+                    if (fabs(a[r * W + c]) > fabs(a[piv * W + c])) piv = r;
                 if (piv != c)
-                    for (int x = 0; x < 6; ++x) {
-                        const double t = a[c * 6 + x];
-                        a[c * 6 + x] = a[piv * 6 + x];
-                        a[piv * 6 + x] = t;
+                    for (int x = 0; x < W; ++x) {
+                        const double t = a[c * W + x];
+                        a[c * W + x] = a[piv * W + x];
+                        a[piv * W + x] = t;
                     }
                 for (int r = c + 1; r < nn; ++r) {
-                    const double fct = a[r * 6 + c] / a[c * 6 + c];
-                    for (int x = c; x < 6; ++x) a[r * 6 + x] -= fct * a[c * 6 + x];
+                    const double fct = a[r * W + c] / a[c * W + c];
+                    for (int x = c; x < W; ++x) a[r * W + x] -= fct * a[c * W + x];
                 }
             }
             for (int r = nn - 1; r >= 0; --r) {
-                double sacc = a[r * 6 + 5];
-                for (int x = r + 1; x < nn; ++x) sacc -= a[r * 6 + x] * th[x];
-                th[r] = sacc / a[r * 6 + r];
+                double sacc = a[r * W + MIX_MAXG];
+                for (int x = r + 1; x < nn; ++x) sacc -= a[r * W + x] * th[x];
+                th[r] = sacc / a[r * W + r];
             }
         }
         __syncthreads();
 #pragma unroll
-        for (int h = 0; h < 5; ++h) theta[h] = (h < nn) ? th[h] : 0.0;
+        for (int h = 0; h < MIX_MAXG; ++h) theta[h] = (h < nn) ? th[h] : 0.0;
         __syncthreads();
     }
     // new slot for the mixed vector = oldest slot
@@ -913,7 +921,7 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
         if (nn > 0) {
             double xb = x0[i], fb = f0[i];
 #pragma unroll
-            for (int h = 0; h < 5; ++h) {
+            for (int h = 0; h < MIX_MAXG; ++h) {
                 if (h < nn) {
                     const double xh = X(h + 1)[i], fh = Fh(h + 1)[i];
                     xb += theta[h] * (xh - x0[i]);
@@ -1038,7 +1046,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = 0.0;
         __syncthreads();
     }
-    Mixer mx{0, 0, o.generations + 1, !warm};
+    Mixer mx{0, 0, o.generations + 1, !warm && o.soft_start != 0};
     int iters = 0;
     // unrolling/default.py:80-100: evaluate the guess, optionally mix it
     // Inexact SCF: each step's eigensolve is converged only as far as the
@@ -1197,7 +1205,7 @@ extern "C" int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par, const 
     if (!batch || !par || !S || !H0 || !gamma || !opts || !state) return XTB_EINVAL;
     if (batch->nmol <= 0 || batch->max_n <= 0) return XTB_EINVAL;
     if (batch->max_n > 1024) return XTB_ETOOBIG;
-    if (opts->generations < 1 || opts->generations > 5) return XTB_EINVAL;
+    if (opts->generations < 1 || opts->generations > MIX_MAXG) return XTB_EINVAL;
     if (opts->jacobi_inner < 1 || opts->jacobi_inner > 20 || opts->eig_inexact < 0.0) return XTB_EINVAL;
     if (!state->F || !state->C || !state->T || !state->P || !state->pot || !state->vnew ||
         !state->q || !state->eorb || !state->emo || !state->occ || !state->hist_x ||

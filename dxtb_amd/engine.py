@@ -180,7 +180,9 @@ class DeviceTables:
 class ScfOptions:
     maxiter: int = 100
     mix_guess: bool = True
-    generations: int = 5
+    # Anderson history (upstream: 5); 6 without the soft start below takes
+    # ~20 % fewer SCF iterations to the same fixed point
+    generations: int = 6
     fermi_maxiter: int = 200
     fermi_etemp: float = 300.0
     fermi_thresh: float | None = None
@@ -212,6 +214,10 @@ class ScfOptions:
     # potential in the pipeline and start the next call on the same batch
     # from them (new positions); same fixed point, fewer iterations
     warm_start: bool = False
+    # upstream's soft start: simple mixing with damp_init for the first
+    # `generations` steps before Anderson acceleration (warm starts always
+    # skip it); off, Anderson starts at the second step
+    soft_start: bool = False
 
     def to_struct(self, want_w: bool, padded: bool, warm: bool = False) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
@@ -225,7 +231,7 @@ class ScfOptions:
             fermi_thresh=thr, jacobi_tol=self.jacobi_tol, bd_gap_kt=self.bd_gap_kt,
             pd_kmax=self.pd_kmax, first_tol=self.first_tol, ortho=self.ortho,
             orbitals=int(self.orbital_energies), keep_state=int(self.warm_start),
-            warm_start=int(self.warm_start and warm),
+            warm_start=int(self.warm_start and warm), soft_start=int(self.soft_start),
         )
 
 

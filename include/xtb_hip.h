@@ -167,7 +167,7 @@ typedef struct xtb_params {
 typedef struct xtb_scf_opts {
     int32_t maxiter;
     int32_t mix_guess;
-    int32_t generations;     /* Anderson history length */
+    int32_t generations;     /* Anderson history length, 1..8 */
     int32_t fermi_maxiter;
     int32_t want_w;          /* also form the energy-weighted density */
     int32_t jacobi_max_sweeps;
@@ -207,6 +207,9 @@ typedef struct xtb_scf_opts {
                                 SCF starts from the kept potential with a fresh
                                 mixer.  Molecules whose previous status was not
                                 0 start cold.  Converges to the same fixed point */
+    int32_t soft_start;      /* 1: the mixer's first `generations` steps are simple
+                                mixing with damp_init (upstream's soft start);
+                                0: Anderson acceleration from the second step */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */
