@@ -240,7 +240,7 @@ def main() -> int:
     ap.add_argument("--lookahead", type=int, default=4,
                     help="chunks the serving loop holds (waiting or in flight) at most")
     ap.add_argument("--hw-queues", type=int, default=12,
-                    help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-32)")
+                    help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-16)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=600)
     ap.add_argument("--latency-requests", type=int, default=600)
@@ -266,8 +266,10 @@ def main() -> int:
 
     # more hardware queues let more micro-batches run side by side (one
     # stream per queue); it must be set before HIP initialises
-    if not 1 <= a.hw_queues <= 32:
-        print("--hw-queues must be in 1..32", file=sys.stderr)
+    # each queue gets its own scratch reservation for the SCF kernel's
+    # stack: 24 queues ran out of resources at launch
+    if not 1 <= a.hw_queues <= 16:
+        print("--hw-queues must be in 1..16", file=sys.stderr)
         return 2
     os.environ["GPU_MAX_HW_QUEUES"] = str(a.hw_queues)
     import gc

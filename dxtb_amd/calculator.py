@@ -69,8 +69,8 @@ def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
         raise NotImplementedError(f"scp_mode '{scp}': only 'potential' is implemented")
     if rest.pop("damp_dynamic", False):
         raise NotImplementedError("damp_dynamic is not implemented")
-    if not 1 <= int(o.generations) <= 8:
-        raise ValueError("damp_generations must be in 1..8")
+    if not 1 <= int(o.generations) <= 6:
+        raise ValueError("damp_generations must be in 1..6")
     return o, rest
 
 

@@ -813,7 +813,10 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
 // a ring buffer of G = generations+1 slots.  Returns the convergence flag of
 // the step (mixer/base.py:229-256 on the step's residual).
 // ---------------------------------------------------------------------------
-constexpr int MIX_MAXG = 8;  // largest history (generations) the mixer takes
+// largest history (generations) the mixer takes; its Gram system's 27 sums
+// fit one pass of the reduction scratch, and 8 generations (44 sums) grow
+// the kernel's stack by ~0.9 KB per lane for ~1 % fewer iterations
+constexpr int MIX_MAXG = 6;
 
 struct Mixer {
     int step, head, G;
@@ -870,9 +873,8 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
 #pragma unroll
             for (int h = 0; h < MIX_MAXG; ++h) acc[NA + h] += d[h] * fi;
         }
-        // the reduction scratch holds 256 doubles: 32 sums (x 8 waves) per pass
-        wg_sum_part<NACC, 0, 32>(acc, sh.red);
-        wg_sum_part<NACC, 32, NACC - 32>(acc, sh.red);
+        static_assert(NACC * NWAVE <= 256, "one pass of the reduction scratch");
+        wg_sum_n<NACC>(acc, sh.red);
         // solve (a * diag(1 + off^2)) theta = b by Gaussian elimination with
         // partial pivoting; one thread, system held in LDS
         constexpr int W = MIX_MAXG + 1;

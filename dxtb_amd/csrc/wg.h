@@ -90,29 +90,6 @@ __device__ inline void wg_sum_n(double (&vals)[K], ldsf* red) {
     __syncthreads();
 }
 
-// sums over the workgroup of vals[K0 .. K0+KN) (KN * NWAVE <= 256 doubles
-// of red); the other entries are left alone
-template <int K, int K0, int KN>
-__device__ inline void wg_sum_part(double (&vals)[K], ldsf* red) {
-    static_assert(K0 + KN <= K && KN * NWAVE <= 256, "reduction scratch");
-    const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
-#pragma unroll
-    for (int k = 0; k < KN; ++k) vals[K0 + k] = wave_sum(vals[K0 + k]);
-    __syncthreads();
-    if (l == 0) {
-#pragma unroll
-        for (int k = 0; k < KN; ++k) red[k * NWAVE + w] = vals[K0 + k];
-    }
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < KN; ++k) {
-        double s = 0.0;
-        for (int i = 0; i < NWAVE; ++i) s += red[k * NWAVE + i];
-        vals[K0 + k] = s;
-    }
-    __syncthreads();
-}
-
 // ---------------------------------------------------------------------------
 // FP64 matrix cores.  v_mfma_f64_16x16x4_f64: D(16x16) += A(16x4) B(4x16);
 // lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds

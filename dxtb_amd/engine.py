@@ -235,11 +235,46 @@ class ScfOptions:
         )
 
 
+class HostArena:
+    """A pinned host buffer that successive launches on one stream reuse:
+    the layout upload, the positions and the staged results of a launch are
+    carved from it.  New pinned allocations synchronise the whole device in
+    the HIP runtime, so a serving loop allocates them once, not per launch.
+    Reuse is safe once the stream's previous launch has been retired (its
+    copies are complete)."""
+
+    def __init__(self, nbytes: int = 16 << 20):
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, pin_memory=True)
+        self.off = 0
+
+    @staticmethod
+    def need(layout: "BatchLayout") -> int:
+        """Upper bound of the bytes one launch on `layout` takes."""
+        nat = int(layout.nat.sum())
+        lay = sum(a.nbytes + 256 for a in layout.arrays.values()) + 4 * layout.B + 256
+        return lay + 2 * (24 * nat + 256) + 8 * layout.B * _lib.OUT_STRIDE + 1024
+
+    def reserve(self, nbytes: int) -> None:
+        """Start a launch: grow (one new pinned allocation) if it would not fit."""
+        if nbytes > self.buf.numel():
+            self.buf = torch.empty(max(nbytes, 2 * self.buf.numel()), dtype=torch.uint8, pin_memory=True)
+        self.off = 0
+
+    def take(self, nbytes: int) -> torch.Tensor:
+        n = (nbytes + 255) // 256 * 256
+        if self.off + n > self.buf.numel():
+            raise RuntimeError("host arena exhausted: reserve() the launch's need first")
+        v = self.buf[self.off:self.off + nbytes]
+        self.off += n
+        return v
+
+
 class Pipeline:
     """Device buffers for one batch and the calls into the HIP library."""
 
-    def __init__(self, layout: BatchLayout, dtables: DeviceTables, device):
+    def __init__(self, layout: BatchLayout, dtables: DeviceTables, device, arena: HostArena | None = None):
         self.L = layout
+        self.arena = arena
         self.dt = dtables
         self.device = torch.device(device)
         self.lib = _lib.load()
@@ -259,10 +294,16 @@ class Pipeline:
             arrays[k] = a
             offs[k] = total
             total += (a.nbytes + 255) // 256 * 256
-        host = np.zeros(max(total, 256), dtype=np.uint8)
+        total = max(total, 256)
+        if arena is not None:
+            self._staging = arena.take(total)
+            host = self._staging.numpy()
+        else:
+            host = np.zeros(total, dtype=np.uint8)
         for k, a in arrays.items():
             host[offs[k]:offs[k] + a.nbytes] = a.view(np.uint8).reshape(-1)
-        self._staging = torch.from_numpy(host).pin_memory()
+        if arena is None:
+            self._staging = torch.from_numpy(host).pin_memory()
         buf = self._staging.to(dev, non_blocking=True)
         tdt = {np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
                np.dtype(np.float64): torch.float64}
@@ -349,6 +390,31 @@ class Pipeline:
         if forces:
             self.run_forces(stream)
 
+    def stage_results(self, stream=None) -> None:
+        """Queue copies of the result records and (after a run with forces)
+        the gradient into pinned host buffers behind the launches on
+        `stream`: once the stream passes them, ``host_records`` and
+        ``host_gradient`` hold the results with no further device access."""
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        def host(n):
+            if self.arena is None:
+                return torch.empty(n, dtype=torch.float64, pin_memory=True)
+            return self.arena.take(8 * n).view(torch.float64)
+
+        with torch.cuda.stream(st):
+            self.h_out = host(self.out.numel())
+            self.h_out.copy_(self.out, non_blocking=True)
+            self.h_grad = None
+            if self._have_w:
+                self.h_grad = host(self.grad.numel())
+                self.h_grad.copy_(self.grad, non_blocking=True)
+
+    def host_records(self) -> np.ndarray:
+        return self.h_out.numpy().reshape(self.L.B, _lib.OUT_STRIDE)
+
+    def host_gradient(self) -> np.ndarray | None:
+        return None if self.h_grad is None else self.h_grad.numpy().reshape(-1, 3)
+
     def run_forces(self, stream=None) -> None:
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         s = ct.c_void_p(st.cuda_stream)
@@ -419,7 +485,7 @@ class Pipeline:
         molecule (iterations + 1) x (9 1/3 n^3 eigen + 2 n^2 k density)
         + 5 1/3 n^3 orthogonaliser + 2 n^2 k energy-weighted density, with
         k = occupied orbitals (alpha count) and the measured iterations."""
-        rec = self.records().cpu().numpy()
+        rec = self.records().cpu().numpy() if getattr(self, "h_out", None) is None else self.host_records()
         n = self.L.n.astype(np.float64)
         k = self.L.nel_a.astype(np.float64)
         it = rec[:, _lib.OUT_ITER] + 1.0

@@ -35,7 +35,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from .engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
+from .engine import BatchLayout, DeviceTables, HostArena, Pipeline, ScfOptions
 from .param import SpeciesTables, species_shells
 
 DEFAULT_EDGES = (48, 96, 160, 256, 400, 1024)
@@ -82,8 +82,11 @@ class ServerStats:
 
 def default_streams() -> int:
     """One stream per hardware queue of the process but the default
-    stream's: GPU_MAX_HW_QUEUES (HIP's default 4) minus one."""
-    return max(1, int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) - 1)
+    stream's: GPU_MAX_HW_QUEUES (HIP's default 4) minus one, at most 15.
+    The runtime reserves scratch memory per queue for the SCF kernel's
+    stack (~2 KB per lane); with 24 queues its launches failed with
+    HSA_STATUS_ERROR_OUT_OF_RESOURCES."""
+    return min(15, max(1, int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) - 1))
 
 
 @functools.lru_cache(maxsize=None)
@@ -117,6 +120,8 @@ class RaggedServer:
         if streams is None:
             streams = default_streams()
         self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
+        # one pinned host arena per stream (a stream holds one launch at a time)
+        self.arenas = {id(s): HostArena() for s in self.streams}
         self.free_streams = deque(self.streams)
         self.edges = tuple(edges)
         self.max_batch = tuple(max_batch)
@@ -180,11 +185,16 @@ class RaggedServer:
             return False
         stream = self.free_streams.popleft()
         th = time.perf_counter()
+        arena = self.arenas[id(stream)]
+        arena.reserve(HostArena.need(lay))
         with torch.cuda.stream(stream):
-            pipe = Pipeline(lay, self.dtables, self.device)
-            pos = torch.from_numpy(np.concatenate([r.positions for r in take]).astype(np.float64))
-            pos = pos.pin_memory().to(self.device, non_blocking=True)
+            pipe = Pipeline(lay, self.dtables, self.device, arena=arena)
+            nat = int(lay.nat.sum())
+            hpos = arena.take(24 * nat).view(torch.float64)
+            np.concatenate([r.positions for r in take], out=hpos.numpy().reshape(nat, 3))
+            pos = hpos.view(nat, 3).to(self.device, non_blocking=True)
             pipe.run(pos, self.opts, forces=self.forces, stream=stream, timing=self.timing)
+            pipe.stage_results(stream)
             ev = torch.cuda.Event()
             ev.record(stream)
         self.stats.host_admit_s += time.perf_counter() - th
@@ -201,17 +211,18 @@ class RaggedServer:
         if self.timing:
             ev0, ev1 = pipe.scf_events
             self.kernel_log.append((ev0.elapsed_time(ev1), pipe.scf_flops()))
-        rec = pipe.records().cpu().numpy()
-        e = pipe.energies().cpu().numpy()
-        g = pipe.gradient().cpu().numpy() if self.forces else None
+        rec = pipe.host_records()
+        e = rec[:, _lib.OUT_ESCF] + rec[:, _lib.OUT_FENERGY] + rec[:, _lib.OUT_EREP]
+        g = pipe.host_gradient() if self.forces else None
+        fo = None if g is None else -g
+        a0 = pipe.L.atom0
         now = time.perf_counter()
         for b, r in enumerate(f.reqs):
             r.energy = float(e[b])
             r.status = int(rec[b, 6])
             r.iterations = int(rec[b, 5])
-            if g is not None:
-                a0 = int(pipe.L.atom0[b])
-                r.forces = -g[a0:a0 + len(r.numbers)].copy()
+            if fo is not None:
+                r.forces = fo[a0[b]:a0[b] + len(r.numbers)]
             r.done = now
             self.results[r.rid] = r
             self.stats.latencies.append(now - r.arrival)

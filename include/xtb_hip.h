@@ -167,7 +167,7 @@ typedef struct xtb_params {
 typedef struct xtb_scf_opts {
     int32_t maxiter;
     int32_t mix_guess;
-    int32_t generations;     /* Anderson history length, 1..8 */
+    int32_t generations;     /* Anderson history length, 1..6 */
     int32_t fermi_maxiter;
     int32_t want_w;          /* also form the energy-weighted density */
     int32_t jacobi_max_sweeps;
