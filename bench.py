@@ -15,8 +15,9 @@ also carries
 * p50/p99 latency of the serving loop under Poisson arrivals at 70 % of the
   measured saturation rate;
 * a roofline entry for the dominant kernel (the persistent SCF kernel,
-  FP64): algorithmic flops per launch / mean launch duration from HIP
-  events, in a pass where launches do not overlap;
+  FP64): algorithmic flops / duration from HIP events of one launch over
+  as many requests as the serving loop holds (`--lookahead` x `--chunk`),
+  alone on the GPU;
 * the CPU baseline: the oracle (a PyTorch FP64 restatement of the upstream
   algorithm) timed on this host on a bounded prefix of the same stream.
 
@@ -199,27 +200,50 @@ def device_peaks(device) -> dict:
 
 
 def roofline_pass(stream, start: int, count: int, device, opts) -> dict:
-    """The SCF kernel alone: one stream, so launches do not overlap, HIP
-    events around every k_scf launch on the stream it runs on; algorithmic
-    flops per launch from the measured iteration counts."""
-    from dxtb_amd.serving import RaggedServer
+    """The SCF kernel alone in one launch: `count` requests of the stream
+    (as many as the serving loop holds) in one ragged batch, one workgroup
+    per molecule, largest first, so that the dispatcher fills CUs as they
+    free up; one stream, HIP events around the k_scf launch on that stream,
+    energy + forces settings; algorithmic flops from the measured iteration
+    counts."""
+    import torch
 
-    rs = RaggedServer(device, opts=opts, streams=1, timing=True)
-    rs.run_stream(_requests(stream, start, count), realtime=False)
-    ms = np.array([x[0] for x in rs.kernel_log])
-    fl = np.array([x[1] for x in rs.kernel_log])
-    achieved = float(fl.mean() / (ms.mean() * 1e-3) / 1e12)
+    from dxtb_amd import _lib
+    from dxtb_amd.engine import BatchLayout, DeviceTables, Pipeline
+    from dxtb_amd.param import SpeciesTables
+    from dxtb_amd.serving import _process_streams
+
+    reqs = _requests(stream, start, count)
+    tb = SpeciesTables((1, 6, 7, 8))
+    lay = BatchLayout([r.numbers for r in reqs], tb, padded=False, chrg=np.zeros(count))
+    # one of the serving loop's process streams: every stream beyond the
+    # hardware queues shares a queue, and later serving runs slowed down
+    st = _process_streams(torch.device(device), 1)[0]
+    with torch.cuda.stream(st):
+        pipe = Pipeline(lay, DeviceTables(tb, device), device)
+        pos = torch.as_tensor(np.concatenate([r.positions for r in reqs]), dtype=torch.float64).to(device)
+        pipe.run(pos, opts, forces=True, stream=st, timing=True)
+    st.synchronize()
+    ev0, ev1 = pipe.scf_events
+    ms = float(ev0.elapsed_time(ev1))
+    fl = pipe.scf_flops()
+    rec = pipe.records().cpu().numpy()
+    bad = int((rec[:, _lib.OUT_STATUS] != 0).sum())
+    achieved = fl / (ms * 1e-3) / 1e12
     # the same flops over the workgroups' own device time (one workgroup per
     # molecule, one CU each): the kernel's efficiency while resident,
-    # independent of how many CUs a launch occupies
+    # independent of how many CUs are busy
     per_cu_peak = FP64_PEAK_TFLOPS * 1e12 / 256.0
-    resident = float(fl.sum() / (rs.stats.scf_busy_s * per_cu_peak))
-    return {"achieved": achieved, "launches": int(len(ms)), "mean_launch_ms": float(ms.mean()),
-            "flops_per_launch": float(fl.mean()), "resident_frac": resident}
+    busy = float(rec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
+    longest = float(rec[:, _lib.OUT_T_TOTAL].max()) * 1e-3
+    return {"achieved": achieved, "launches": 1, "mean_launch_ms": ms, "flops_per_launch": fl,
+            "molecules": count, "requests": f"{start}-{start + count - 1}", "nonconverged": bad,
+            "resident_frac": fl / (busy * per_cu_peak), "cu_occupancy": busy / (256.0 * ms * 1e-3),
+            "longest_molecule_ms": longest}
 
 
-def measured_traffic(chunk: int, warmup: int) -> float | None:
-    """HBM bytes per k_scf launch of this roofline pass, from the rocprofv3
+def measured_traffic(start: int, count: int) -> float | None:
+    """HBM bytes of the roofline pass's k_scf launch, from the rocprofv3
     FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) summarised in
     profiles/k_scf_traffic.json; None when the workload differs."""
     f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_scf_traffic.json")
@@ -228,7 +252,7 @@ def measured_traffic(chunk: int, warmup: int) -> float | None:
             t = json.load(fh)
     except (OSError, ValueError):
         return None
-    if t.get("chunk") != chunk or t.get("warmup") != warmup or t.get("seed") != 4:
+    if t.get("start") != start or t.get("count") != count or t.get("seed") != 4:
         return None
     return float(t["traffic_bytes_per_launch"])
 
@@ -236,14 +260,14 @@ def measured_traffic(chunk: int, warmup: int) -> float | None:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--lookahead", type=int, default=4,
                     help="chunks the serving loop holds (waiting or in flight) at most")
     ap.add_argument("--hw-queues", type=int, default=12,
                     help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-16)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=600)
-    ap.add_argument("--latency-requests", type=int, default=600)
+    ap.add_argument("--latency-requests", type=int, default=2000)
     ap.add_argument("--padded-batch", type=int, default=64)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -299,7 +323,7 @@ def main() -> int:
     if a.profile_run:
         # the roofline pass alone (for rocprofv3 runs): its mean launch time is
         # the figure the kernel trace's average duration of k_scf must match
-        rp = roofline_pass(stream, a.warmup * a.chunk, a.chunk, dev, opts)
+        rp = roofline_pass(stream, a.warmup * a.chunk, a.lookahead * a.chunk, dev, opts)
         print(json.dumps({"profile_run": rp}))
         return 0
     server = RaggedServer(dev, opts=opts)
@@ -357,11 +381,12 @@ def main() -> int:
         barrier = a.steps * a.chunk / (time.perf_counter() - tb0)
         log(f"ragged, chunks drained one at a time: {barrier:.1f} molecules/s")
 
-    # roofline of the SCF kernel: one stream, launches do not overlap
-    rp = roofline_pass(stream, a.warmup * a.chunk, a.chunk, dev, opts)
+    # roofline of the SCF kernel: one launch over the requests the loop holds
+    rp = roofline_pass(stream, a.warmup * a.chunk, a.lookahead * a.chunk, dev, opts)
     achieved = rp["achieved"]
-    log(f"scf kernel: {rp['launches']} launches, mean {rp['mean_launch_ms']:.2f} ms, "
-        f"{achieved:.3f} TFLOP/s")
+    log(f"scf kernel: one launch of {rp['molecules']} molecules, {rp['mean_launch_ms']:.2f} ms, "
+        f"{achieved:.3f} TFLOP/s, CU occupancy {rp['cu_occupancy']:.3f}, "
+        f"longest molecule {rp['longest_molecule_ms']:.1f} ms")
 
     # latency under Poisson arrivals at 70 % of the saturation rate
     rate = 0.7 * value
@@ -371,9 +396,19 @@ def main() -> int:
     for r in lat_reqs:
         t += rng.exponential(1.0 / rate)
         r.arrival = t
+    # the serving loop (same process streams), with the results of the runs
+    # above released and the heap frozen again, as after start-up
+    server.results.clear()
+    served = held = None
+    if barrier is not None:
+        bsrv.results.clear()
+    gc.collect()
+    gc.freeze()
     ls = RaggedServer(dev, opts=opts)
     lat = ls.run_stream(lat_reqs, realtime=True)
-    log(f"latency @ {rate:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms")
+    log(f"latency @ {rate:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms "
+        f"({lat['launches']} launches, host admission {ls.stats.host_admit_s:.3f} s, "
+        f"{lat['seconds']:.3f} s for {len(lat_reqs)} requests)")
 
     padded = None
     if not a.no_padded:
@@ -415,10 +450,14 @@ def main() -> int:
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                     "traffic": measured_traffic(a.chunk, a.warmup), "kernel": "xtb::k_scf",
+                     "traffic": measured_traffic(a.warmup * a.chunk, a.lookahead * a.chunk),
+                     "kernel": "xtb::k_scf",
+                     "workload": f"one launch: requests {rp['requests']} of the stream, largest first",
                      "launches": rp["launches"], "mean_launch_ms": round(rp["mean_launch_ms"], 3),
                      "flops_per_launch": rp["flops_per_launch"],
                      "resident_frac": round(rp["resident_frac"], 4),
+                     "cu_occupancy": round(rp["cu_occupancy"], 4),
+                     "longest_molecule_ms": round(rp["longest_molecule_ms"], 2),
                      "measured_peaks": peaks},
         "cpu_baseline": cpu,
     }

@@ -28,6 +28,7 @@ OUT_ESCF, OUT_EH0, OUT_EES2, OUT_EES3, OUT_FENERGY = 0, 1, 2, 3, 4
 OUT_ITER, OUT_STATUS, OUT_EF_A, OUT_EF_B, OUT_EREP, OUT_SWEEPS = 5, 6, 7, 8, 9, 10
 OUT_T_TOTAL = 14  # SCF kernel device time of the molecule, microseconds
 OUT_T_START = 20  # device wall clock at the molecule's start, microseconds
+OUT_XCD = 22  # XCD the molecule's workgroup ran on
 
 P = ct.c_void_p
 i32 = ct.c_int32
@@ -68,6 +69,7 @@ class XtbScfOpts(ct.Structure):
         ("diag_offset", f64), ("fermi_thresh", f64), ("jacobi_tol", f64), ("eig_inexact", f64),
         ("bd_gap_kt", f64), ("pd_kmax", f64), ("first_tol", f64), ("ortho", i32), ("orbitals", i32),
         ("keep_state", i32), ("warm_start", i32), ("soft_start", i32),
+        ("wg_skip", i32),
     ]
 
 

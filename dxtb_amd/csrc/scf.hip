@@ -945,6 +945,9 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const Shared& sh,
 __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const double* S,
                                             const double* H0, const double* gam,
                                             xtb_scf_opts o, xtb_scf_state st, int big_cap) {
+    // leading workgroups that only shift the molecules' workgroups to later
+    // XCDs (the whole workgroup leaves before any barrier)
+    if ((int)blockIdx.x < o.wg_skip) return;
     extern __shared__ double lds[];
     const Lay L = make_lay(b.max_n, b.max_nsh, b.max_nat);
     Shared sh;
@@ -964,7 +967,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     sh.big_cap = big_cap;
 
     Mol M;
-    M.m = st.order[blockIdx.x];
+    M.m = st.order[blockIdx.x - o.wg_skip];
     M.n = b.mol_n[M.m];
     M.nao = b.mol_nao[M.m];
     M.nsh = b.mol_nsh[M.m];
@@ -1168,6 +1171,10 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_T_TOTAL] = now_us() - tstart;
         out[XTB_OUT_T_START] = tstart;
         out[XTB_OUT_T_PD] = tacc[3];
+        // hwreg(HW_REG_XCC_ID, 0, 4) and hwreg(HW_REG_HW_ID, 0, 32): where the
+        // workgroup ran, for the placement of launches across the XCDs
+        out[XTB_OUT_XCD] = (double)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15);
+        out[XTB_OUT_HW_ID] = (double)(unsigned)__builtin_amdgcn_s_getreg(4 | (0 << 6) | (31 << 11));
 #ifdef XTB_PD_TIMERS
         for (int i = 0; i < 10; ++i) out[22 + i] = ptacc[i];
 #endif
@@ -1208,6 +1215,7 @@ extern "C" int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par, const 
     if (batch->nmol <= 0 || batch->max_n <= 0) return XTB_EINVAL;
     if (batch->max_n > 1024) return XTB_ETOOBIG;
     if (opts->generations < 1 || opts->generations > MIX_MAXG) return XTB_EINVAL;
+    if (opts->wg_skip < 0 || opts->wg_skip > 7) return XTB_EINVAL;
     if (opts->jacobi_inner < 1 || opts->jacobi_inner > 20 || opts->eig_inexact < 0.0) return XTB_EINVAL;
     if (!state->F || !state->C || !state->T || !state->P || !state->pot || !state->vnew ||
         !state->q || !state->eorb || !state->emo || !state->occ || !state->hist_x ||
@@ -1219,7 +1227,7 @@ extern "C" int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par, const 
     if (hipFuncSetAttribute((const void*)k_scf, hipFuncAttributeMaxDynamicSharedMemorySize, bytes) !=
         hipSuccess)
         return XTB_EHIP;
-    hipLaunchKernelGGL(k_scf, dim3(batch->nmol), dim3(NT), bytes, (hipStream_t)stream, *batch, *par,
+    hipLaunchKernelGGL(k_scf, dim3(batch->nmol + opts->wg_skip), dim3(NT), bytes, (hipStream_t)stream, *batch, *par,
                        S, H0, gamma, *opts, *state, cap);
     return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP;
 }

@@ -219,7 +219,7 @@ class ScfOptions:
     # skip it); off, Anderson starts at the second step
     soft_start: bool = False
 
-    def to_struct(self, want_w: bool, padded: bool, warm: bool = False) -> _lib.XtbScfOpts:
+    def to_struct(self, want_w: bool, padded: bool, warm: bool = False, wg_skip: int = 0) -> _lib.XtbScfOpts:
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
         return _lib.XtbScfOpts(
             maxiter=self.maxiter, mix_guess=int(self.mix_guess), generations=self.generations,
@@ -232,6 +232,7 @@ class ScfOptions:
             pd_kmax=self.pd_kmax, first_tol=self.first_tol, ortho=self.ortho,
             orbitals=int(self.orbital_energies), keep_state=int(self.warm_start),
             warm_start=int(self.warm_start and warm), soft_start=int(self.soft_start),
+            wg_skip=int(wg_skip),
         )
 
 
@@ -349,11 +350,12 @@ class Pipeline:
         )
 
     def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None,
-            timing: bool = False) -> None:
+            timing: bool = False, wg_skip: int = 0) -> None:
         """positions: (nat_total, 3) float64 on the device, bohr.
 
         With ``timing`` HIP events bracket the SCF launch on its stream
-        (``self.scf_events``) for kernel-duration measurements."""
+        (``self.scf_events``) for kernel-duration measurements.  ``wg_skip``
+        (0..7) shifts the SCF launch's workgroups to later XCDs."""
         if positions.dtype != torch.float64 or not positions.is_contiguous():
             raise ValueError("positions must be contiguous float64")
         if positions.shape != (int(self.L.nat.sum()), 3):
@@ -373,7 +375,7 @@ class Pipeline:
         _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
         # a warm start needs the state a previous keep_state launch on these
         # buffers left (in stream order, so the launches need no other sync)
-        o = opts.to_struct(want_w=forces, padded=self.L.padded, warm=self._kept)
+        o = opts.to_struct(want_w=forces, padded=self.L.padded, warm=self._kept, wg_skip=wg_skip)
         self._kept = False
         self.state = self._state()
         if timing:

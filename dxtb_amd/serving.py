@@ -40,6 +40,7 @@ from .param import SpeciesTables, species_shells
 
 DEFAULT_EDGES = (48, 96, 160, 256, 400, 1024)
 DEFAULT_MAXB = (512, 512, 384, 256, 192, 128)
+DEFAULT_CLASS_TIME = (2.0e-3, 4.7e-3, 12e-3, 22e-3, 57e-3, 143e-3)  # seconds per molecule
 
 
 @dataclass
@@ -64,6 +65,7 @@ class _Flight:
     event: torch.cuda.Event
     bytes: int
     t_launch: float
+    klass: int  # size class of the launch's requests
 
 
 @dataclass
@@ -89,6 +91,22 @@ def default_streams() -> int:
     return min(15, max(1, int(os.environ.get("GPU_MAX_HW_QUEUES", "4")) - 1))
 
 
+_STREAMS: dict = {}
+
+
+def _process_streams(device: torch.device, count: int) -> list:
+    """The process's serving streams on `device`, created once and shared by
+    every server.  Each new HIP stream beyond the hardware queues shares a
+    queue with an earlier one, and a second server on fresh streams
+    measured 300-500 ms p50 latency where the first one's streams gave
+    ~45 ms (same load)."""
+    key = device.index if device.index is not None else torch.cuda.current_device()
+    have = _STREAMS.setdefault(key, [])
+    while len(have) < count:
+        have.append(torch.cuda.Stream(device=device))
+    return have[:count]
+
+
 @functools.lru_cache(maxsize=None)
 def _orbital_table() -> np.ndarray:
     """Orbitals per element, indexed by atomic number (-1: unsupported)."""
@@ -112,6 +130,7 @@ class RaggedServer:
     def __init__(self, device="cuda:0", opts: ScfOptions | None = None, streams: int | None = None,
                  edges=DEFAULT_EDGES, max_batch=DEFAULT_MAXB, hbm_budget_gb: float = 64.0,
                  elements=(1, 6, 7, 8), forces: bool = True, max_wait_s: float = 0.002,
+                 wait_frac: float = 0.5, max_wait_cap_s: float = 0.1, xcd_rotate: bool = True,
                  timing: bool = False):
         self.device = torch.device(device)
         self.opts = opts or ScfOptions()
@@ -119,7 +138,7 @@ class RaggedServer:
         self.dtables = DeviceTables(self.tables, self.device)
         if streams is None:
             streams = default_streams()
-        self.streams = [torch.cuda.Stream(device=self.device) for _ in range(streams)]
+        self.streams = _process_streams(self.device, streams)
         # one pinned host arena per stream (a stream holds one launch at a time)
         self.arenas = {id(s): HostArena() for s in self.streams}
         self.free_streams = deque(self.streams)
@@ -130,6 +149,23 @@ class RaggedServer:
         self.budget = int(hbm_budget_gb * 2**30)
         self.forces = forces
         self.max_wait = max_wait_s
+        # a class waits for more requests up to wait_frac of its molecules'
+        # mean SCF time (measured on retired launches; capped): a launch
+        # holds its stream until its slowest molecule is done, so launches
+        # of one or two large molecules every few milliseconds would tie up
+        # every stream while most CUs idle
+        self.wait_frac = wait_frac
+        self.max_wait_cap = max_wait_cap_s
+        # prior until launches of the class retire: the median per-molecule
+        # SCF time of the default classes on MI355X (energy + forces, 1e-10
+        # charge tolerance); other edges ~1.5 ms + 1.1 ns x (1.25 lower edge)^3
+        if self.edges == DEFAULT_EDGES:
+            self.class_time = list(DEFAULT_CLASS_TIME)
+        else:
+            lo = (0,) + self.edges[:-1]
+            self.class_time = [1.5e-3 + 1.1e-9 * (1.25 * max(e, 20)) ** 3 for e in lo]
+        self.xcd_rotate = xcd_rotate
+        self._xcd_next = 0
         self.stats = ServerStats()
         self.results: dict[int, Request] = {}
         self.timing = timing
@@ -157,7 +193,7 @@ class RaggedServer:
             if not q:
                 continue
             full = len(q) >= self.max_batch[k]
-            waited = now - q[0].arrival >= self.max_wait
+            waited = now - q[0].arrival >= self._wait_for(k)
             if not (eager or full or waited or not self.inflight):
                 continue
             # oldest request first; among requests that arrived at the same
@@ -166,6 +202,10 @@ class RaggedServer:
             if best is None or q[0].arrival <= best_t:
                 best, best_t = k, q[0].arrival
         return best
+
+    def _wait_for(self, k: int) -> float:
+        t = self.class_time[k]
+        return min(self.max_wait_cap, max(self.max_wait, self.wait_frac * t))
 
     def _admit(self, now: float, eager: bool = False) -> bool:
         if not self.free_streams:
@@ -193,12 +233,15 @@ class RaggedServer:
             hpos = arena.take(24 * nat).view(torch.float64)
             np.concatenate([r.positions for r in take], out=hpos.numpy().reshape(nat, 3))
             pos = hpos.view(nat, 3).to(self.device, non_blocking=True)
-            pipe.run(pos, self.opts, forces=self.forces, stream=stream, timing=self.timing)
+            # launches start on the XCD after the previous launch's last one
+            pipe.run(pos, self.opts, forces=self.forces, stream=stream, timing=self.timing,
+                     wg_skip=self._xcd_next if self.xcd_rotate else 0)
+            self._xcd_next = (self._xcd_next + len(take)) % 8
             pipe.stage_results(stream)
             ev = torch.cuda.Event()
             ev.record(stream)
         self.stats.host_admit_s += time.perf_counter() - th
-        self.inflight.append(_Flight(pipe, take, stream, ev, nbytes, th))
+        self.inflight.append(_Flight(pipe, take, stream, ev, nbytes, th, k))
         self.stats.launches += 1
         self.stats.peak_bytes = max(self.stats.peak_bytes, self._inflight_bytes())
         return True
@@ -228,6 +271,8 @@ class RaggedServer:
             self.stats.latencies.append(now - r.arrival)
         self.stats.completed += len(f.reqs)
         self.stats.scf_busy_s += float(rec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
+        t = float(rec[:, _lib.OUT_T_TOTAL].mean()) * 1e-6
+        self.class_time[f.klass] = 0.8 * self.class_time[f.klass] + 0.2 * t
         if self.timing:
             self.stats.spans += list(zip(rec[:, _lib.OUT_T_START], rec[:, _lib.OUT_T_TOTAL]))
             # host admit time, retire time, molecules, max n, device first start, last end (us)

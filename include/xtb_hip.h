@@ -96,6 +96,8 @@ extern "C" {
 #define XTB_OUT_SWEEPS_LAST 19   /* Jacobi sweeps of the final (exact) step */
 #define XTB_OUT_T_START 20  /* device wall clock at the molecule's start, microseconds */
 #define XTB_OUT_T_PD 21     /* perturbative decoupling steps (part of T_EIGEN) */
+#define XTB_OUT_XCD 22      /* accelerator complex die (XCD) the molecule's workgroup ran on */
+#define XTB_OUT_HW_ID 23    /* the workgroup's HW_ID register (CU, shader array, engine) */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {
@@ -210,6 +212,12 @@ typedef struct xtb_scf_opts {
     int32_t soft_start;      /* 1: the mixer's first `generations` steps are simple
                                 mixing with damp_init (upstream's soft start);
                                 0: Anderson acceleration from the second step */
+    int32_t wg_skip;         /* 0..7: the launch starts with this many workgroups
+                                that exit at once.  Workgroup i of a launch runs
+                                on XCD (i + first) mod 8 where the dispatcher
+                                mostly starts each launch at XCD 0, so launches
+                                of a few molecules pile up on the first XCDs;
+                                a serving loop rotates this to spread them */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */
