@@ -263,7 +263,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=40)
     ap.add_argument("--lookahead", type=int, default=4,
                     help="chunks the serving loop holds (waiting or in flight) at most")
-    ap.add_argument("--hw-queues", type=int, default=12,
+    ap.add_argument("--hw-queues", type=int, default=16,
                     help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-16)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=600)
@@ -388,27 +388,34 @@ def main() -> int:
         f"{achieved:.3f} TFLOP/s, CU occupancy {rp['cu_occupancy']:.3f}, "
         f"longest molecule {rp['longest_molecule_ms']:.1f} ms")
 
-    # latency under Poisson arrivals at 70 % of the saturation rate
-    rate = 0.7 * value
-    lat_reqs = _requests(stream, (a.warmup + a.steps) * a.chunk, a.latency_requests)
-    rng = np.random.default_rng(44)
-    t = 0.0
-    for r in lat_reqs:
-        t += rng.exponential(1.0 / rate)
-        r.arrival = t
-    # the serving loop (same process streams), with the results of the runs
-    # above released and the heap frozen again, as after start-up
+    # latency under Poisson arrivals at 70 % of the saturation rate (and at
+    # 50 %, for the shape of the curve): the serving loop on the same process
+    # streams, with the results of the runs above released and the heap
+    # frozen again, as after start-up
     server.results.clear()
     served = held = None
     if barrier is not None:
         bsrv.results.clear()
     gc.collect()
     gc.freeze()
-    ls = RaggedServer(dev, opts=opts)
-    lat = ls.run_stream(lat_reqs, realtime=True)
-    log(f"latency @ {rate:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms "
-        f"({lat['launches']} launches, host admission {ls.stats.host_admit_s:.3f} s, "
-        f"{lat['seconds']:.3f} s for {len(lat_reqs)} requests)")
+    curve = []
+    for frac in (0.5, 0.7):
+        r_ = frac * value
+        lat_reqs = _requests(stream, (a.warmup + a.steps) * a.chunk, a.latency_requests)
+        rng = np.random.default_rng(44)
+        t = 0.0
+        for r in lat_reqs:
+            t += rng.exponential(1.0 / r_)
+            r.arrival = t
+        ls = RaggedServer(dev, opts=opts)
+        lat = ls.run_stream(lat_reqs, realtime=True)
+        log(f"latency @ {r_:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms "
+            f"({lat['launches']} launches, host admission {ls.stats.host_admit_s:.3f} s, "
+            f"{lat['seconds']:.3f} s for {len(lat_reqs)} requests)")
+        curve.append({"rate_per_s": round(r_, 1), "p50_ms": round(lat["p50_ms"], 2),
+                      "p99_ms": round(lat["p99_ms"], 2)})
+        ls.results.clear()
+    rate = curve[-1]["rate_per_s"]
 
     padded = None
     if not a.no_padded:
@@ -444,8 +451,8 @@ def main() -> int:
             "padded_dropin_molecules_per_s": None if padded is None else round(padded, 3),
             "ragged_over_padded": None if padded is None else round(value / padded, 3),
             "padded_batch": a.padded_batch,
-            "latency_rate_per_s": round(rate, 2), "p50_ms": round(lat["p50_ms"], 2),
-            "p99_ms": round(lat["p99_ms"], 2), "nonconverged": bad,
+            "latency_rate_per_s": round(rate, 2), "p50_ms": curve[-1]["p50_ms"],
+            "p99_ms": curve[-1]["p99_ms"], "latency_curve": curve, "nonconverged": bad,
             "forces_over_energy": fc, "md_warm_start": md,
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,

@@ -84,3 +84,31 @@ def test_config2_forces_finite_differences_and_oracle():
         z, x = mols[b]
         r = og.singlepoint(z, x, opts=TIGHT_ORACLE, with_forces=True)
         assert np.abs(g[int(a0[b]):int(a0[b]) + len(z)] - r["gradient"].numpy()).max() < 1e-7
+
+
+def test_command_line_matches_the_calculator(capsys):
+    """`python -m dxtb_amd` on upstream's H2O coord and NO2 xyz files (one
+    padded batch, --grad, JSON output) prints the calculator's energies and
+    gradient; the batch holds an open shell (NO2, 23 electrons)."""
+    import json
+    from pathlib import Path
+
+    from dxtb_amd import cli, io
+
+    g = Path(__file__).resolve().parent / "golden" / "mols"
+    print("case: command line, H2O coord", flush=True)
+    assert cli.main([str(g / "H2O.coord"), "--grad", "--json"]) == 0
+    out = json.loads(capsys.readouterr().out.strip().splitlines()[-1])
+    z, p = io.read(g / "H2O.coord")
+    calc = GFN1Calculator(torch.as_tensor(z, device="cuda:0"), device="cuda:0", dtype=torch.float64)
+    pos = torch.as_tensor(p, dtype=torch.float64, device="cuda:0").requires_grad_(True)
+    f = calc.get_forces(pos, 0)
+    e = calc.get_energy(pos.detach(), 0)
+    assert abs(out["energy"] - float(e)) < 1e-12
+    assert np.abs(np.array(out["gradient"]) + f.detach().cpu().numpy()).max() < 1e-12
+    r = og.singlepoint(z, p, opts=TIGHT_ORACLE)
+    assert abs(out["energy"] - float(r["energy"])) < 1e-8
+    print("case: command line, two files as a batch", flush=True)
+    assert cli.main([str(g / "H2O.coord"), str(g / "NO2.xyz"), "--forces"]) == 0
+    text = capsys.readouterr().out
+    assert text.count("total energy") == 2 and text.count("Forces") == 2
