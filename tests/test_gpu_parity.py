@@ -298,3 +298,23 @@ def test_mixer_settings_reach_the_same_fixed_point(fx):
         assert int(r.records()[:, 6].abs().max()) == 0
     it = [float(r.records()[:, 5].mean()) for r in runs]
     assert it[1] < it[0]
+
+
+def test_xcd_shift_changes_placement_not_results(fx):
+    """`wg_skip` leading workgroups exit at once and move every molecule's
+    workgroup to later XCDs: the results are those of the unshifted launch
+    (the forces kernels accumulate with atomics, hence the 1e-12)."""
+    from helpers import TIGHT, native_pipeline
+
+    mols = _mols(fx)
+    print("case: XCD shift 0 vs 5", flush=True)
+    a = run_native(mols, forces=True)
+    pipe, _ = native_pipeline(mols)
+    pos = torch.tensor(np.concatenate([p for _, p in mols]), dtype=torch.float64, device="cuda:0")
+    pipe.run(pos, ScfOptions(**TIGHT), forces=True, wg_skip=5)
+    torch.cuda.synchronize()
+    assert np.abs(a.energies().cpu().numpy() - pipe.energies().cpu().numpy()).max() < 1e-12
+    assert np.abs(a.gradient().cpu().numpy() - pipe.gradient().cpu().numpy()).max() < 1e-12
+    ra, rb = a.records().cpu().numpy(), pipe.records().cpu().numpy()
+    assert np.array_equal(ra[:, 5], rb[:, 5]) and (rb[:, 6] == 0).all()
+    assert ((rb[:, 22] >= 0) & (rb[:, 22] < 8)).all()

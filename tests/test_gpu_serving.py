@@ -78,3 +78,21 @@ def test_calculator_api_semantics():
     calc.reset()
     calc.get_energy(p2).sum().backward()
     assert (p2.grad + f).abs().max().item() < 1e-12
+
+
+def test_staged_results_equal_plain_copies():
+    """Results copied to pinned memory behind each launch (stage_results)
+    equal those copied at retirement."""
+    stream = W.config4(seed=12, count=30)
+    print("case: serving, staged vs plain result copies, 30 requests seed 12", flush=True)
+    out = []
+    for staged in (False, True):
+        reqs = [Request(i, z, p, arrival=0.0) for i, (_, z, p) in enumerate(stream)]
+        srv = RaggedServer("cuda:0", opts=ScfOptions(**OPTS), streams=3, stage_results=staged)
+        srv.run_stream(reqs, realtime=False)
+        out.append(srv.results)
+    for rid, r in out[0].items():
+        s = out[1][rid]
+        assert r.status == 0 and s.status == 0
+        assert abs(r.energy - s.energy) < 1e-12
+        assert np.abs(r.forces - s.forces).max() < 1e-12

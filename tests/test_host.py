@@ -179,3 +179,16 @@ def test_pair_scaling_derivatives_match_finite_differences():
                 hh[name[0]] += s * h
             ks.append(shell_pair_k(tb, hh, ee))
         assert np.abs((ks[0] - ks[1]) / (2 * h) - dk).max() < 1e-8, name
+
+
+def test_scf_rejects_an_out_of_range_xcd_shift():
+    """xtb_scf validates wg_skip (0..7) before it touches the device."""
+    lib = L.load()
+    b = L.XtbBatch(nmol=1, max_n=4)
+    p = L.XtbParams()
+    st = L.XtbScfState(**{f: 8 for f, _ in L.XtbScfState._fields_})
+    from dxtb_amd.engine import ScfOptions
+
+    o = ScfOptions().to_struct(want_w=False, padded=False, wg_skip=8)
+    rc = lib.xtb_scf(ct.byref(b), ct.byref(p), 8, 8, 8, ct.byref(o), ct.byref(st), None)
+    assert rc == 1  # invalid argument
