@@ -13,7 +13,8 @@ also carries
 * the plain padded-batch drop-in on the same requests (arrival order,
   `--padded-batch` molecules per padded batch) and the ragged/padded ratio;
 * p50/p99 latency of the serving loop under Poisson arrivals at 70 % of the
-  measured saturation rate;
+  measured saturation rate (the JSON's p50/p99), and at 50 %
+  (`latency_curve`);
 * a roofline entry for the dominant kernel (the persistent SCF kernel,
   FP64): algorithmic flops / duration from HIP events of one launch over
   as many requests as the serving loop holds (`--lookahead` x `--chunk`),
