@@ -279,7 +279,7 @@ def main() -> int:
     ap.add_argument("--profile-run", action="store_true",
                     help="serve one chunk only (for rocprofv3 runs), print nothing else")
     ap.add_argument("--profile-serving", action="store_true",
-                    help="serve one chunk through the 4-stream loop only (kernel timelines)")
+                    help="only serve 8 chunks through the continuous-batching loop (kernel traces)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -329,9 +329,24 @@ def main() -> int:
         return 0
     server = RaggedServer(dev, opts=opts)
     if a.profile_serving:
-        server.run_stream(_requests(stream, 0, a.chunk), realtime=False)  # warm-up
-        r = server.run_stream(_requests(stream, a.chunk, a.chunk), realtime=False)
-        print(json.dumps({"profile_serving": r}))
+        # the continuous-batching loop over 8 chunks, after one warm-up chunk
+        server.run_stream(_requests(stream, 0, a.chunk), realtime=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        held, served = [], []
+        for k in range(8):
+            reqs = _requests(stream, (k + 1) * a.chunk, a.chunk)
+            server.feed(reqs)
+            held.append(reqs)
+            served += reqs
+            if len(held) >= a.lookahead:
+                server.serve_until(held.pop(0))
+        server.serve_until(served)
+        torch.cuda.synchronize()
+        sec = time.perf_counter() - t0
+        print(json.dumps({"profile_serving": {"molecules": len(served), "seconds": sec,
+                                              "molecules_per_s": len(served) / sec,
+                                              "launches": server.stats.launches}}))
         return 0
     pos = 0
     for w in range(a.warmup):

@@ -40,7 +40,7 @@ class XtbBatch(ct.Structure):
     _fields_ = [
         ("nmol", i32), ("nat_total", i32), ("nsh_total", i32), ("nvec_total", i32),
         ("max_n", i32), ("max_nsh", i32), ("max_nat", i32), ("ntile_sh", i32),
-        ("ntile_at", i32), ("pad0", i32), ("nmat_total", i64), ("ngam_total", i64),
+        ("ntile_at", i32), ("has_padding", i32), ("nmat_total", i64), ("ngam_total", i64),
         ("mol_nat", P), ("mol_nsh", P), ("mol_nao", P), ("mol_n", P), ("mol_atom0", P),
         ("mol_sh0", P), ("mol_vec0", P), ("mol_mat0", P), ("mol_gam0", P), ("mol_nel", P),
         ("atom_sp", P), ("atom_mol", P), ("atom_sh0", P), ("atom_nsh", P),

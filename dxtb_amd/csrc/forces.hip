@@ -19,10 +19,13 @@
 
 namespace xtb {
 
-__global__ void k_zero(double* __restrict__ a, int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        a[i] = 0.0;
+// zero two arrays in one small launch: every workgroup of a launch waits for
+// a free CU, which under a busy serving loop (CUs held by SCF workgroups)
+// costs milliseconds per workgroup spread, so this is one workgroup
+__global__ void __launch_bounds__(1024) k_zero2(double* __restrict__ a, int64_t na,
+                                                double* __restrict__ b, int64_t nb) {
+    for (int64_t i = threadIdx.x; i < na; i += blockDim.x) a[i] = 0.0;
+    for (int64_t i = threadIdx.x; i < nb; i += blockDim.x) b[i] = 0.0;
 }
 
 #define XTB_MAX_US_LDS_F 64
@@ -179,8 +182,8 @@ extern "C" int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
     if (!state->P || !state->T || !state->pot || !state->qsh) return XTB_EINVAL;
     if (par->nus > XTB_MAX_US_LDS_F) return XTB_ETOOBIG;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero, dim3(64), dim3(256), 0, s, grad, (int64_t)3 * batch->nat_total);
-    hipLaunchKernelGGL(k_zero, dim3(16), dim3(256), 0, s, dedcn, (int64_t)batch->nat_total);
+    hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1024), 0, s, grad, (int64_t)3 * batch->nat_total, dedcn,
+                       (int64_t)batch->nat_total);
     if (batch->ntile_at > 0) {
         hipLaunchKernelGGL(k_grad_pairs, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
                            positions, cn, gamma, *state, dedcn, grad);

@@ -242,8 +242,10 @@ extern "C" int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
     if (batch->ntile_sh > 0)
         hipLaunchKernelGGL(k_s_h0, dim3(batch->ntile_sh), dim3(256), 0, s, *batch, *par, positions,
                            cn, S, H0);
-    // padding orbitals (padded layout); molecules without padding return at once
-    hipLaunchKernelGGL(k_pad_fill, dim3(8, batch->nmol), dim3(256), 0, s, *batch, S, H0);
+    // padding orbitals (padded layout only: every workgroup of a launch waits
+    // for a free CU, which under a busy serving loop costs milliseconds)
+    if (batch->has_padding)
+        hipLaunchKernelGGL(k_pad_fill, dim3(8, batch->nmol), dim3(256), 0, s, *batch, S, H0);
     return check_launch();
 }
 

@@ -335,7 +335,7 @@ class Pipeline:
         self.batch = _lib.XtbBatch(
             nmol=L.B, nat_total=natt, nsh_total=int(L.sh0[-1]), nvec_total=nv,
             max_n=int(L.n.max()), max_nsh=int(L.nsh.max()), max_nat=int(L.nat.max()),
-            ntile_sh=L.ntile_sh, ntile_at=L.ntile_at, pad0=0, nmat_total=nm,
+            ntile_sh=L.ntile_sh, ntile_at=L.ntile_at, has_padding=int((L.n > L.nao).any()), nmat_total=nm,
             ngam_total=L.ngam_total,
             **{k: a[k].data_ptr() for k in (
                 "mol_nat", "mol_nsh", "mol_nao", "mol_n", "mol_atom0", "mol_sh0", "mol_vec0",

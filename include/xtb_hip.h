@@ -110,7 +110,7 @@ typedef struct xtb_batch {
     int32_t max_nat;     /* max of mol_nat */
     int32_t ntile_sh;    /* entries of the shell-pair tile list */
     int32_t ntile_at;    /* entries of the atom-pair tile list */
-    int32_t pad0;
+    int32_t has_padding; /* 1 if some molecule has mol_n > mol_nao (padded layout) */
     int64_t nmat_total;  /* doubles of one packed n x n buffer */
     int64_t ngam_total;  /* doubles of the packed nsh x nsh buffer */
     /* per molecule [nmol] */
