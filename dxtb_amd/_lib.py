@@ -107,6 +107,8 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_build_s_h0.argtypes = [bp, pp, P, P, P, P, P]
     lib.xtb_build_gamma.restype = i32
     lib.xtb_build_gamma.argtypes = [bp, pp, P, P, P]
+    lib.xtb_build_s_h0_gamma.restype = i32
+    lib.xtb_build_s_h0_gamma.argtypes = [bp, pp, P, P, P, P, P, P]
     lib.xtb_scf.restype = i32
     lib.xtb_scf.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfOpts), ct.POINTER(XtbScfState), P]
     lib.xtb_scf_lds_bytes.restype = i32
@@ -134,7 +136,7 @@ def load(path: str | os.PathLike | None = None):
 EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
-    "xtb_param_grad_basis",
+    "xtb_param_grad_basis", "xtb_build_s_h0_gamma",
 )
 
 

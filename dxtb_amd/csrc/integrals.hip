@@ -46,6 +46,30 @@ __device__ inline void stage_prims(const xtb_params& par, double* s_alpha, doubl
     __syncthreads();
 }
 
+// one upper-triangle entry (i <= j < nsh) of molecule m's gamma, mirrored
+__device__ inline void gamma_pair(const xtb_batch& b, const xtb_params& par,
+                                  const double* __restrict__ pos, double* __restrict__ gam,
+                                  int m, int i, int j, int nsh) {
+    const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
+    const double eps = 2.220446049250313e-16;
+    const int ga = at0 + b.sh_atom[sh0 + i], gb = at0 + b.sh_atom[sh0 + j];
+    double d;
+    if (ga == gb) d = eps;
+    else {
+        const double dx = pos[3 * ga] - pos[3 * gb], dy = pos[3 * ga + 1] - pos[3 * gb + 1],
+                     dz = pos[3 * ga + 2] - pos[3 * gb + 2];
+        const double r = sqrt(dx * dx + dy * dy + dz * dz) + eps;
+        d = r * r;
+    }
+    const double hi = par.us_hard[b.sh_us[sh0 + i]] + eps;
+    const double hj = par.us_hard[b.sh_us[sh0 + j]] + eps;
+    const double avg = 2.0 / (1.0 / hi + 1.0 / hj);
+    const double g = 1.0 / sqrt(d + 1.0 / (avg * avg));
+    const size_t g0 = (size_t)b.mol_gam0[m];
+    gam[g0 + (size_t)i * nsh + j] = g;
+    gam[g0 + (size_t)j * nsh + i] = g;
+}
+
 // ---------------------------------------------------------------------------
 // S and H0 (src/dxtb/_src/integral/driver/pytorch/impls/overlap.py:147-243,
 // src/dxtb/_src/xtb/base.py:251-360).  Off-site blocks:
@@ -57,7 +81,8 @@ __device__ inline void stage_prims(const xtb_params& par, double* s_alpha, doubl
 __global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
                                               const double* __restrict__ pos,
                                               const double* __restrict__ cn,
-                                              double* __restrict__ S, double* __restrict__ H0) {
+                                              double* __restrict__ S, double* __restrict__ H0,
+                                              double* __restrict__ gam) {
     __shared__ double s_alpha[XTB_MAX_US_LDS * XTB_MAXPRIM];
     __shared__ double s_coeff[XTB_MAX_US_LDS * XTB_MAXPRIM];
     stage_prims(par, s_alpha, s_coeff);
@@ -111,6 +136,8 @@ __global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
             S[e2] = blk[x][y];
             H0[e2] = hblk[x][y];
         }
+    // the same shell pair's ES2 entry (one launch fewer per batch)
+    if (gam) gamma_pair(b, par, pos, gam, m, i, j, nsh);
 }
 
 // Padding orbitals of the padded layout: unit diagonal in S (as the
@@ -144,24 +171,7 @@ __global__ void __launch_bounds__(256) k_gamma(xtb_batch b, xtb_params par,
     const int j = tl[2] + threadIdx.x % 16;
     const int nsh = b.mol_nsh[m];
     if (i >= nsh || j >= nsh || j < i) return;
-    const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
-    const double eps = 2.220446049250313e-16;
-    const int ga = at0 + b.sh_atom[sh0 + i], gb = at0 + b.sh_atom[sh0 + j];
-    double d;
-    if (ga == gb) d = eps;
-    else {
-        const double dx = pos[3 * ga] - pos[3 * gb], dy = pos[3 * ga + 1] - pos[3 * gb + 1],
-                     dz = pos[3 * ga + 2] - pos[3 * gb + 2];
-        const double r = sqrt(dx * dx + dy * dy + dz * dz) + eps;
-        d = r * r;
-    }
-    const double hi = par.us_hard[b.sh_us[sh0 + i]] + eps;
-    const double hj = par.us_hard[b.sh_us[sh0 + j]] + eps;
-    const double avg = 2.0 / (1.0 / hi + 1.0 / hj);
-    const double g = 1.0 / sqrt(d + 1.0 / (avg * avg));
-    const size_t g0 = (size_t)b.mol_gam0[m];
-    gam[g0 + (size_t)i * nsh + j] = g;
-    gam[g0 + (size_t)j * nsh + i] = g;
+    gamma_pair(b, par, pos, gam, m, i, j, nsh);
 }
 
 // ---------------------------------------------------------------------------
@@ -230,9 +240,8 @@ extern "C" int32_t xtb_workspace_doubles(const xtb_batch* batch, int32_t generat
     return XTB_OK;
 }
 
-extern "C" int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
-                                  const double* positions, double* S, double* H0, double* cn,
-                                  void* stream) {
+static int32_t build_s_h0(const xtb_batch* batch, const xtb_params* par, const double* positions,
+                          double* S, double* H0, double* cn, double* gamma, void* stream) {
     if (!batch || !par || !positions || !S || !H0 || !cn) return XTB_EINVAL;
     if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
     if (par->nus > XTB_MAX_US_LDS) return XTB_ETOOBIG;
@@ -241,12 +250,25 @@ extern "C" int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
                        positions, cn);
     if (batch->ntile_sh > 0)
         hipLaunchKernelGGL(k_s_h0, dim3(batch->ntile_sh), dim3(256), 0, s, *batch, *par, positions,
-                           cn, S, H0);
+                           cn, S, H0, gamma);
     // padding orbitals (padded layout only: every workgroup of a launch waits
     // for a free CU, which under a busy serving loop costs milliseconds)
     if (batch->has_padding)
         hipLaunchKernelGGL(k_pad_fill, dim3(8, batch->nmol), dim3(256), 0, s, *batch, S, H0);
     return check_launch();
+}
+
+extern "C" int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
+                                  const double* positions, double* S, double* H0, double* cn,
+                                  void* stream) {
+    return build_s_h0(batch, par, positions, S, H0, cn, nullptr, stream);
+}
+
+extern "C" int32_t xtb_build_s_h0_gamma(const xtb_batch* batch, const xtb_params* par,
+                                        const double* positions, double* S, double* H0,
+                                        double* cn, double* gamma, void* stream) {
+    if (!gamma) return XTB_EINVAL;
+    return build_s_h0(batch, par, positions, S, H0, cn, gamma, stream);
 }
 
 extern "C" int32_t xtb_build_gamma(const xtb_batch* batch, const xtb_params* par,

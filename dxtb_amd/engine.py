@@ -372,9 +372,9 @@ class Pipeline:
             self.hist_x = torch.empty((g + 1) * self.L.nvec_total, dtype=torch.float64, device=self.device)
             self.hist_f = torch.empty_like(self.hist_x)
         pos = positions.data_ptr()
-        _lib.check(lib.xtb_build_s_h0(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
-                                      self.cn.data_ptr(), s), "xtb_build_s_h0")
-        _lib.check(lib.xtb_build_gamma(b, p, pos, self.gamma.data_ptr(), s), "xtb_build_gamma")
+        _lib.check(lib.xtb_build_s_h0_gamma(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
+                                            self.cn.data_ptr(), self.gamma.data_ptr(), s),
+                   "xtb_build_s_h0_gamma")
         _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
         # a warm start needs the state a previous keep_state launch on these
         # buffers left (in stream order, so the launches need no other sync)

@@ -13,6 +13,7 @@
  *                    derivatives; replaces ES2.get_cache / ES3.get_cache
  *                    (src/dxtb/_src/components/interactions/coulomb/
  *                     secondorder.py:216-276,799-870; thirdorder.py:182-250)
+ *   xtb_build_s_h0_gamma  both of the above in one pass (one launch fewer)
  *   xtb_scf          persistent per-molecule SCF (Fock build, generalized
  *                    eigensolve, Fermi smearing, density, Mulliken charges,
  *                    ES2/ES3 potential, Anderson mixing, energies); replaces
@@ -259,6 +260,14 @@ int32_t xtb_build_s_h0(const xtb_batch* batch, const xtb_params* par,
 
 int32_t xtb_build_gamma(const xtb_batch* batch, const xtb_params* par,
                         const double* positions, double* gamma, void* stream);
+
+/* xtb_build_s_h0 and xtb_build_gamma in one pass over the shell-pair tiles
+ * (the same results; one launch fewer per batch).  Replaces the same two
+ * upstream call sites as the pair: GFN1Hamiltonian.build
+ * (_src/xtb/base.py:251-360) and ES2.get_cache (coulomb/secondorder.py:216-276). */
+int32_t xtb_build_s_h0_gamma(const xtb_batch* batch, const xtb_params* par,
+                             const double* positions, double* S, double* H0,
+                             double* cn, double* gamma, void* stream);
 
 /* Dynamic LDS the SCF kernel will request for this batch, and the part of
  * it available to the eigensolver (doubles).  Host-only. */

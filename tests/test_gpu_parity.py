@@ -323,3 +323,32 @@ def test_xcd_shift_changes_placement_not_results(fx):
     ra, rb = a.records().cpu().numpy(), pipe.records().cpu().numpy()
     assert np.array_equal(ra[:, 5], rb[:, 5]) and (rb[:, 6] == 0).all()
     assert ((rb[:, 22] >= 0) & (rb[:, 22] < 8)).all()
+
+
+def test_fused_integrals_equal_the_separate_entry_points(fx):
+    """xtb_build_s_h0_gamma (what the pipeline calls) writes exactly what
+    xtb_build_s_h0 and xtb_build_gamma write."""
+    import ctypes as ct
+
+    from dxtb_amd import _lib
+
+    mols = _mols(fx)
+    print("case: fused S/H0/gamma vs separate entry points", flush=True)
+    p = run_native(mols, forces=False)
+    S, H0, gam, cn = (torch.full_like(t, float("nan")) for t in (p.S, p.H0, p.gamma, p.cn))
+    lib = _lib.load()
+    s = ct.c_void_p(torch.cuda.current_stream().cuda_stream)
+    pos = p.positions.data_ptr()
+    _lib.check(lib.xtb_build_s_h0(ct.byref(p.batch), ct.byref(p.dt.s), pos, S.data_ptr(), H0.data_ptr(),
+                                  cn.data_ptr(), s), "xtb_build_s_h0")
+    _lib.check(lib.xtb_build_gamma(ct.byref(p.batch), ct.byref(p.dt.s), pos, gam.data_ptr(), s),
+               "xtb_build_gamma")
+    torch.cuda.synchronize()
+    assert torch.equal(cn, p.cn)
+    for b in range(len(mols)):
+        # each molecule's blocks (the packed buffers keep alignment gaps
+        # between molecules that no kernel writes)
+        assert torch.equal(p.matrix(S, b), p.matrix(p.S, b))
+        assert torch.equal(p.matrix(H0, b), p.matrix(p.H0, b))
+        g0, m = int(p.L.gam0[b]), int(p.L.nsh[b])
+        assert torch.equal(gam[g0:g0 + m * m], p.gamma[g0:g0 + m * m])
