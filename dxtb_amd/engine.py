@@ -217,6 +217,10 @@ class ScfOptions:
     # potential in the pipeline and start the next call on the same batch
     # from them (new positions); same fixed point, fewer iterations
     warm_start: bool = False
+    # warm starts from the second on: start from the potential extrapolated
+    # linearly from the last two calls (for smooth trajectories: MD with a
+    # small time step, optimisation paths); the fixed point is the same
+    warm_extrapolate: bool = False
     # upstream's soft start: simple mixing with damp_init for the first
     # `generations` steps before Anderson acceleration (warm starts always
     # skip it); off, Anderson starts at the second step
@@ -326,6 +330,7 @@ class Pipeline:
         self.qsh = z(int(L.sh0[-1]))
         self.out = torch.zeros(L.B * _lib.OUT_STRIDE, dtype=torch.float64, device=dev)
         self._kept = False  # the state buffers hold orbitals and potential to warm-start from
+        self._pot_prev = None  # kept potential of the call before the last (extrapolation)
         self._have_w = False  # the last run formed the energy-weighted density
         self.grad = z(3 * natt)
         self.dedcn = z(natt)
@@ -378,6 +383,16 @@ class Pipeline:
         _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
         # a warm start needs the state a previous keep_state launch on these
         # buffers left (in stream order, so the launches need no other sync)
+        if self._kept and opts.warm_extrapolate:
+            # start from 2 v_(k-1) - v_(k-2): along a smooth trajectory the
+            # potential moves about linearly from step to step
+            with torch.cuda.stream(st):
+                last = self.pot.clone()
+                if self._pot_prev is not None:
+                    self.pot.mul_(2.0).sub_(self._pot_prev)
+                self._pot_prev = last
+        elif not self._kept:
+            self._pot_prev = None
         o = opts.to_struct(want_w=forces, padded=self.L.padded, warm=self._kept, wg_skip=wg_skip)
         self._kept = False
         self.state = self._state()

@@ -352,3 +352,26 @@ def test_fused_integrals_equal_the_separate_entry_points(fx):
         assert torch.equal(p.matrix(H0, b), p.matrix(p.H0, b))
         g0, m = int(p.L.gam0[b]), int(p.L.nsh[b])
         assert torch.equal(gam[g0:g0 + m * m], p.gamma[g0:g0 + m * m])
+
+
+def test_extrapolated_warm_start_reaches_the_same_fixed_point(fx):
+    """warm_extrapolate starts each call from 2 v_(k-1) - v_(k-2) of the kept
+    potentials: along a smooth trajectory (constant drift) every geometry's
+    results equal a cold start's to the parity tolerances."""
+    from helpers import TIGHT, native_pipeline
+
+    rng = np.random.default_rng(13)
+    m0 = [W.chain(rng, 30), W.water_cluster(rng, 20), mol(fx, "lys_xao")]
+    drift = [0.01 * rng.standard_normal(p.shape) for _, p in m0]
+    geo = [[(z, p + k * d) for (z, p), d in zip(m0, drift)] for k in range(4)]
+    print("case: extrapolated warm start along a 4-geometry drift", flush=True)
+    pipe, run = native_pipeline(geo[0])
+    o = ScfOptions(**TIGHT, warm_start=True, warm_extrapolate=True)
+    for g, mols in enumerate(geo):
+        run(mols, o, forces=True)
+        c = run_native(mols, forces=True)
+        rw, rc = pipe.records().cpu().numpy(), c.records().cpu().numpy()
+        assert np.abs(rw[:, 6]).max() == 0
+        assert np.abs(pipe.energies().cpu().numpy() - c.energies().cpu().numpy()).max() < 1e-9
+        assert np.abs(pipe.gradient().cpu().numpy() - c.gradient().cpu().numpy()).max() < 1e-8
+        print(f"geometry {g}: iterations extrapolated warm {rw[:, 5]} cold {rc[:, 5]}", flush=True)
