@@ -60,7 +60,7 @@ def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
         ("fermi_maxiter", "fermi_maxiter"), ("fermi_thresh", "fermi_thresh"),
         # not an upstream option: successive calls on the same molecules
         # start from the previous call's orbitals and potential
-        ("warm_start", "warm_start"),
+        ("warm_start", "warm_start"), ("warm_extrapolate", "warm_extrapolate"),
     ):
         if key in rest:
             setattr(o, attr, rest.pop(key))
