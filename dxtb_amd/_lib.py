@@ -22,6 +22,7 @@ MOL_MAXITER = 1
 MOL_FERMI = 2
 MOL_NONFINITE = 4
 MOL_JACOBI = 8
+MOL_OVERLAP = 16
 
 OUT_STRIDE = 32
 OUT_ESCF, OUT_EH0, OUT_EES2, OUT_EES3, OUT_FENERGY = 0, 1, 2, 3, 4
@@ -115,6 +116,8 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_scf_lds_bytes.argtypes = [bp, ct.POINTER(i32), ct.POINTER(i32)]
     lib.xtb_repulsion.restype = i32
     lib.xtb_repulsion.argtypes = [bp, pp, P, P, P]
+    lib.xtb_repulsion_gradient.restype = i32
+    lib.xtb_repulsion_gradient.argtypes = [bp, pp, P, P, P]
     lib.xtb_forces.restype = i32
     lib.xtb_forces.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P, P]
     lib.xtb_param_grad.restype = i32
@@ -136,7 +139,7 @@ def load(path: str | os.PathLike | None = None):
 EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
-    "xtb_param_grad_basis", "xtb_build_s_h0_gamma",
+    "xtb_param_grad_basis", "xtb_build_s_h0_gamma", "xtb_repulsion_gradient",
 )
 
 

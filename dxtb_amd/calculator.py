@@ -115,8 +115,10 @@ class GFN1Calculator:
         self.force_convergence = bool(opts.pop("force_convergence", False))
         exclude = opts.pop("exclude", [])
         self.exclude = {exclude} if isinstance(exclude, str) else set(exclude)
+        # upstream's result cache is off by default (constants/defaults.py:266)
+        self.cache_enabled = bool(opts.pop("cache_enabled", False))
         for k in ("scf_mode", "mixer", "int_driver", "int_level", "int_uplo", "batch_mode",
-                  "method", "strict", "guess", "f_atol", "fermi_partition", "cache_enabled",
+                  "method", "strict", "guess", "f_atol", "fermi_partition",
                   "max_element", "anomaly", "grad", "skip_compat_checks", "int_cutoff"):
             opts.pop(k, None)
         self.scf_opts, rest = _opts_to_scf(opts)
@@ -185,16 +187,34 @@ class GFN1Calculator:
             self._pipe_key = key
         return self._pipe
 
+    def _cached(self, positions: torch.Tensor, ckey, forces: bool, orbitals: bool):
+        """The cached run for these positions (compared by value with the
+        copy the cache holds), charges and spin, if it computed at least what
+        is asked for; only with cache_enabled, as upstream."""
+        if not self.cache_enabled:
+            return None
+        hit = self._cache.get("run")
+        if hit is None:
+            return None
+        (key, pos, had_forces, had_orb), res = hit
+        if key != ckey or (forces and not had_forces) or (orbitals and not had_orb):
+            return None
+        p = positions.detach()
+        if pos.shape != p.shape or not torch.equal(pos, p):
+            return None
+        return res
+
+    def _store(self, positions: torch.Tensor, ckey, forces: bool, orbitals: bool, res: dict) -> None:
+        if self.cache_enabled:
+            self._cache["run"] = ((ckey, positions.detach().clone(), forces, orbitals), res)
+
     def _run(self, positions: torch.Tensor, chrg=0, spin=None, forces: bool = False,
              orbitals: bool = False) -> dict:
         self._check(positions)
-        ckey = (positions.data_ptr(), positions._version, id(positions),
-                self._charges_array(chrg).tobytes(), str(spin))
-        hit = self._cache.get("run")
-        # a cached run serves any request for no more than it computed
-        if hit is not None and hit[0][0] == ckey and (hit[0][1] or not forces) \
-                and (hit[0][2] or not orbitals):
-            return hit[1]
+        ckey = (self._charges_array(chrg).tobytes(), str(spin))
+        hit = self._cached(positions, ckey, forces, orbitals)
+        if hit is not None:
+            return hit
         if not self.batched:
             from .serving import orbital_count
 
@@ -211,23 +231,36 @@ class GFN1Calculator:
         rec = pipe.records().clone()
         status = rec[:, _lib.OUT_STATUS].to(torch.int64).cpu().numpy()
         self._raise_on_status(status)
-        exclude_rep = bool({"rep"} & self.exclude)
-        exclude_scf = bool({"scf", "all"} & self.exclude)
+        use_scf, use_rep = self._components()
         e = torch.zeros(len(self._mols), dtype=torch.float64, device=self.device)
-        if not exclude_scf:
+        if use_scf:
             e = e + rec[:, _lib.OUT_ESCF] + rec[:, _lib.OUT_FENERGY]
-        if not exclude_rep and "all" not in self.exclude:
+        if use_rep:
             e = e + rec[:, _lib.OUT_EREP]
-        if exclude_rep or exclude_scf:
-            if forces:
-                raise NotImplementedError("forces with excluded components are not implemented")
         res = {"energy": e if self.batched else e[0], "records": rec, "pipeline": pipe}
         if forces:
+            # the forces kernel differentiates E_scf - TS + E_rep; excluded
+            # terms are taken out with the repulsion gradient
+            if use_scf and use_rep:
+                gm = pipe.gradient().clone()
+            elif use_scf:
+                gm = pipe.gradient() - pipe.repulsion_gradient()
+            elif use_rep:
+                gm = pipe.repulsion_gradient()
+            else:
+                gm = torch.zeros_like(pipe.gradient())
             g = torch.zeros(positions.shape, dtype=torch.float64, device=self.device)
-            g[self._mask] = pipe.gradient().clone()
+            g[self._mask] = gm
             res["gradient_full"] = g
-        self._cache["run"] = ((ckey, forces, orbitals), res)
+        self._store(positions, ckey, forces, orbitals, res)
         return res
+
+    def _components(self) -> tuple[bool, bool]:
+        """(electronic energy, repulsion) included in the total; "disp" and
+        "hal" change nothing here (dispersion is not built, halogen bonding
+        is zero for H/C/N/O)."""
+        ex = self.exclude
+        return not ({"scf", "all"} & ex), not ({"rep", "all"} & ex)
 
     def _run_large(self, positions, chrg, spin, forces, ckey) -> dict:
         from .large import LargeSystem
@@ -245,23 +278,46 @@ class GFN1Calculator:
             if self.force_convergence:
                 raise SCFConvergenceError(msg)
             warnings.warn(msg, SCFConvergenceWarning)
-        e = torch.tensor(r["energy"], dtype=torch.float64, device=self.device)
-        if {"rep"} & self.exclude:
-            e = e - r["repulsion"]
+        use_scf, use_rep = self._components()
+        e = torch.tensor((r["energy"] - r["repulsion"] if use_scf else 0.0)
+                         + (r["repulsion"] if use_rep else 0.0), dtype=torch.float64, device=self.device)
         rec = torch.zeros((1, _lib.OUT_STRIDE), dtype=torch.float64, device=self.device)
         rec[0, _lib.OUT_ESCF] = r["scf"]
         rec[0, _lib.OUT_EREP] = r["repulsion"]
         rec[0, _lib.OUT_ITER] = r["iterations"]
         res = {"energy": e, "records": rec, "pipeline": self._pipe.pipe, "large": r}
         if forces:
+            pipe = self._pipe.pipe
+            if use_scf and use_rep:
+                gm = r["gradient"]
+            elif use_scf:
+                gm = r["gradient"] - pipe.repulsion_gradient()
+            elif use_rep:
+                gm = pipe.repulsion_gradient()
+            else:
+                gm = torch.zeros_like(r["gradient"])
             g = torch.zeros(positions.shape, dtype=torch.float64, device=self.device)
-            g[self._mask] = r["gradient"]
+            g[self._mask] = gm
             res["gradient_full"] = g
         # orbital energies are never formed here (get_mo_energies raises)
-        self._cache["run"] = ((ckey, forces, True), res)
+        self._store(positions, ckey, forces, True, res)
         return res
 
     def _raise_on_status(self, status: np.ndarray) -> None:
+        if np.any(status & _lib.MOL_OVERLAP):
+            # upstream's Cholesky reduction of the generalized eigenproblem
+            # (tad-mctc eighb) raises here
+            raise torch.linalg.LinAlgError(
+                "linalg.cholesky: The factorization could not be completed because the overlap "
+                "matrix is not positive-definite (molecules "
+                f"{np.nonzero(status & _lib.MOL_OVERLAP)[0].tolist()}).")
+        if np.any(status & _lib.MOL_JACOBI):
+            msg = ("The eigensolver of the final SCF step reached its sweep limit "
+                   f"(molecules {np.nonzero(status & _lib.MOL_JACOBI)[0].tolist()}); "
+                   "orbitals and energies may be inaccurate.")
+            if self.force_convergence:
+                raise SCFConvergenceError(msg)
+            warnings.warn(msg, SCFConvergenceWarning)
         if np.any(status & _lib.MOL_FERMI):
             raise RuntimeError("Fermi energy failed to converge.")
         if np.any(status & _lib.MOL_NONFINITE):

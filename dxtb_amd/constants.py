@@ -7,7 +7,13 @@ test-suite (``test/test_scf/samples.py``) to 1e-11 Eh.
 
 AA2AU = 1.0 / 0.52917721090380  # Angstrom -> bohr
 EV2AU = 1.0 / 27.21138505  # eV -> Hartree (xTB/tblite value, CODATA 2010)
-KELVIN2AU = 3.166811563455546e-06  # k_B in Hartree/K
+# k_B in Hartree/K, the xTB/tblite value: the tblite free energies of the
+# H, C, N, O atoms and ions in test/test_scf/test_elements_gfn1.py (with
+# fractional p occupations) agree with it to 2e-12 Eh, not with CODATA 2018
+KELVIN2AU = 3.166808578545117e-06
+# below this kT (0.1 K) upstream fills orbitals by the aufbau principle
+# instead of Fermi smearing (src/dxtb/_src/scf/base.py:888-889)
+KT_AUFBAU = 3e-7
 
 # Covalent radii for the D3 coordination number (Pyykkoe & Atsumi 2009,
 # Angstrom; scaled by 4/3 as DFT-D3 does) and atomic radii of the xTB

@@ -213,6 +213,36 @@ __global__ void __launch_bounds__(256) k_repulsion(xtb_batch b, xtb_params par,
         atomicAdd(out + (size_t)m * XTB_OUT_STRIDE + XTB_OUT_EREP, red[0] + red[1] + red[2] + red[3]);
 }
 
+// repulsion gradient (repulsion/base.py:337-406), one thread per atom A
+// summing over the other atoms of its molecule in order (no atomics, so the
+// result is the same bit for bit on every run):
+//   dE/dR_A = sum_B dE_AB/dr (R_A - R_B) / r,
+//   dE_AB/dr = -E_AB (1/r + a kexp r^(kexp-1))
+__global__ void __launch_bounds__(256) k_repulsion_grad(xtb_batch b, xtb_params par,
+                                                        const double* __restrict__ pos,
+                                                        double* __restrict__ grad) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= b.nat_total) return;
+    const int m = b.atom_mol[a];
+    const int a0 = b.mol_atom0[m], na = b.mol_nat[m];
+    const int sa = b.atom_sp[a];
+    const double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
+    double g[3] = {0.0, 0.0, 0.0};
+    for (int j = a0; j < a0 + na; ++j) {
+        if (j == a) continue;
+        const double R[3] = {x - pos[3 * j], y - pos[3 * j + 1], z - pos[3 * j + 2]};
+        const double r = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+        if (r > par.rep_cutoff) continue;
+        const int sb = b.atom_sp[j];
+        const double alpha = sqrt(par.sp_arep[sa] * par.sp_arep[sb] + 2.2250738585072014e-308);
+        const double rk = pow(r, par.kexp);
+        const double e = par.sp_zeff[sa] * par.sp_zeff[sb] * exp(-alpha * rk) / r;
+        const double f = -e * (1.0 / r + alpha * par.kexp * rk / r) / r;
+        for (int k = 0; k < 3; ++k) g[k] += f * R[k];
+    }
+    for (int k = 0; k < 3; ++k) grad[3 * a + k] = g[k];
+}
+
 }  // namespace xtb
 
 using namespace xtb;
@@ -291,5 +321,14 @@ extern "C" int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
     if (batch->ntile_at > 0)
         hipLaunchKernelGGL(k_repulsion, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
                            positions, out);
+    return check_launch();
+}
+
+extern "C" int32_t xtb_repulsion_gradient(const xtb_batch* batch, const xtb_params* par,
+                                          const double* positions, double* grad, void* stream) {
+    if (!batch || !par || !positions || !grad) return XTB_EINVAL;
+    if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
+    hipLaunchKernelGGL(k_repulsion_grad, dim3((batch->nat_total + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, *batch, *par, positions, grad);
     return check_launch();
 }

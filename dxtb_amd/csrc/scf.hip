@@ -177,7 +177,7 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
         Y = W;
         W = oldZ;
     }
-    if (it >= 60) *status |= XTB_MOL_JACOBI;
+    if (it >= 60) *status |= XTB_MOL_OVERLAP;  // S numerically singular
     const double sc = 1.0 / sqrt(c);
     for (size_t e = threadIdx.x; e < nn; e += NT) M.Ct[e] = Z[e] * sc;  // Z may alias Ct
     __syncthreads();
@@ -205,7 +205,8 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
 // X_i = D_i^-1 (E_i - L_i,<i X_<i), again with GEMMs.  A block padded beyond
 // n is the identity.  Scratch: F (up to 32 x n, or mt x 32 with stride n),
 // P (the nb x nb inverse, stride nb).
-// Returns 0; a non-positive pivot sets XTB_MOL_JACOBI in *status.
+// Returns 0; a non-positive pivot (S not positive definite, where upstream's
+// torch.linalg.cholesky raises) sets XTB_MOL_OVERLAP in *status.
 // ---------------------------------------------------------------------------
 __device__ void chol_block_inverse(ldsf* D, ldsf* Di, glbf* Dg, int nb, int lane) {
     // lower-triangular D (NB x NB, LDS) -> Di = D^-1 (LDS) and its leading
@@ -318,7 +319,7 @@ __device__ __noinline__ int chol_setup(const Mol& M, const Shared& sh, int* stat
     for (int r = w; r < n; r += NWAVE)
         for (int c = lane; c < n; c += WAVE) L[(size_t)r * n + c] = M.S[(size_t)r * n + c];
     __syncthreads();
-    if (!chol_factor(L, n, R, Dg, sh)) *status |= XTB_MOL_JACOBI;
+    if (!chol_factor(L, n, R, Dg, sh)) *status |= XTB_MOL_OVERLAP;
     for (int r = w; r < n; r += NWAVE)
         for (int c = lane; c < n; c += WAVE) X[(size_t)r * n + c] = 0.0;
     __syncthreads();
@@ -365,10 +366,19 @@ __device__ __noinline__ bool warm_orthonormalise(const Mol& M, const Shared& sh)
     return true;
 }
 
+// electronic temperatures below this (0.1 K in Hartree) fill the orbitals
+// by the aufbau principle instead of Fermi smearing
+// (src/dxtb/_src/scf/base.py:888-889)
+constexpr double KT_AUFBAU = 3e-7;
+
 // ---------------------------------------------------------------------------
 // Fermi smearing of one spin channel (src/dxtb/_src/wavefunction/
-// filling.py:201-366): mid-gap start, Newton on the electron count, exponent
-// cutoff 50, threshold on |N - sum f|.  Sorted eigenvalues in sh.skey.
+// filling.py:201-366): mid-gap start, Newton on the rounded electron count
+// (scf/base.py:879), exponent cutoff 50, threshold on |N - sum f|.  Below
+// KT_AUFBAU the aufbau occupations of the unrounded count instead
+// (filling.py:116-198; base.py:888-889 keeps the initial filling of
+// iterator.py:147-189): orbital j (from 0) is full while j + 1 < nel, the
+// next one takes nel - ceil(nel) + 1.  Sorted eigenvalues in sh.skey.
 // Writes the sorted occupations to occ_out and adds them to sh.ftot.
 // ---------------------------------------------------------------------------
 __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, double nel, double kt, double thr,
@@ -381,6 +391,19 @@ __device__ __noinline__ bool fermi_channel(const Mol& M, const Shared& sh, doubl
         __syncthreads();
         return true;
     }
+    if (kt < KT_AUFBAU) {
+        const double frac = nel - ceil(nel) + 1.0;
+        for (int j = threadIdx.x; j < n; j += NT) {
+            const double idx = (double)(j + 1);
+            const double f = idx < nel ? 1.0 : (idx < nel + 1.0 ? frac : 0.0);
+            occ_out[j] = f;
+            sh.ftot[sh.sidx[j]] += f;
+        }
+        *efermi = 0.0;
+        __syncthreads();
+        return true;
+    }
+    nel = rint(nel);
     int homo = (int)ceil(nel - 5e-15) - 1;
     if (homo < 0) homo = 0;
     if (homo > n - 1) homo = n - 1;
@@ -663,7 +686,9 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         const double na = M.nel_a, nb = M.nel_b;
         const int ia = (int)rint(na), ib = (int)rint(nb);
         bool ok = fabs(na - ia) < 1e-12 && fabs(nb - ib) < 1e-12 && ib <= ia;
-        const double need = o.bd_gap_kt * o.kt;
+        // (at kT -> 0 a gap of bd_gap_kt x kT(300 K) is still asked for:
+        // the decoupling needs a clear gap to converge, not to be exact)
+        const double need = o.bd_gap_kt * fmax(o.kt, 9.5e-4);
         if (ok && ia > 0 && ia < n) ok = sh.skey[ia] - sh.skey[ia - 1] >= need;
         if (ok && ib > 0 && ib < n) ok = sh.skey[ib] - sh.skey[ib - 1] >= need;
         if (ok) { cb_lo = ib; cb_hi = ia; }
@@ -711,7 +736,9 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         sh.tacc[1] += t1 - t0;
         sh.tacc[2] += t2 - t1;
     }
-    if (sw < 0) *status |= XTB_MOL_JACOBI;
+    // an intermediate step's eigensolve that ran out of sweeps only slows
+    // the SCF down; the final step's results would be unreliable
+    if (sw < 0 && final_step) *status |= XTB_MOL_JACOBI;
     // sort eigenvalues
     for (int j = threadIdx.x; j < sh.npow; j += NT) {
         sh.skey[j] = (j < n) ? sh.ev[j] : INFINITY;
@@ -1016,6 +1043,17 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
     if (!warm) sweeps = o.ortho == 1 ? chol_setup(M, sh, &status) : ortho_setup(M, sh, &status);
     tacc[0] = now_us() - tstart;
     double ef[2] = {0.0, 0.0};
+    if (status & XTB_MOL_OVERLAP) {
+        // no orthogonaliser: report the failure instead of running the SCF on
+        // a meaningless basis (status is uniform over the workgroup)
+        if (threadIdx.x == 0) {
+            for (int k = 0; k < XTB_OUT_STRIDE; ++k)
+                if (k != XTB_OUT_EREP) out[k] = 0.0;
+            out[XTB_OUT_ESCF] = NAN;
+            out[XTB_OUT_STATUS] = (double)status;
+        }
+        return;
+    }
 
     if (warm) {
         // orbital-energy estimates of the kept orbitals in the new Fock

@@ -38,16 +38,29 @@ def _cumsum0(a: np.ndarray) -> np.ndarray:
 
 
 def alpha_beta(nel: np.ndarray, spin) -> tuple[np.ndarray, np.ndarray]:
-    """Alpha/beta electrons (src/dxtb/_src/wavefunction/filling.py:41-113),
-    rounded as the occupation sum is upstream (scf/base.py:880)."""
+    """Alpha/beta electrons (src/dxtb/_src/wavefunction/filling.py:41-113).
+
+    Not rounded: the SCF kernel rounds them for Fermi smearing as upstream
+    rounds the occupation sum (scf/base.py:879) and fills the unrounded
+    counts by the aufbau principle at zero temperature.  A spin that does not
+    fit the electron count raises ValueError, as upstream (:80-93)."""
     nel = np.asarray(nel, dtype=np.float64)
-    uhf = np.zeros_like(nel) if spin is None else np.broadcast_to(np.asarray(spin, dtype=np.float64), nel.shape)
     nr = np.round(nel)
+    if spin is None:
+        uhf = np.zeros_like(nel)
+    else:
+        uhf = np.broadcast_to(np.asarray(spin, dtype=np.float64), nel.shape)
+        if np.any(uhf > nr):
+            raise ValueError(f"Number of unpaired electrons ({uhf}) larger than "
+                             f"number of electrons ({nel}).")
+        if np.any(np.remainder(uhf, 2) != np.remainder(nr, 2)):
+            raise ValueError(f"Odd (even) number of unpaired electrons ({uhf}) but even "
+                             f"(odd) number of electrons ({nel}) given.")
     nuhf = np.where(np.remainder(uhf, 2) == np.remainder(nr, 2), uhf, np.remainder(nr, 2))
     diff = np.minimum(nuhf, nel)
     nb = (nel - diff) / 2.0
     na = nb + diff
-    return np.round(na), np.round(nb)
+    return na, nb
 
 
 @functools.lru_cache(maxsize=None)
@@ -227,6 +240,9 @@ class ScfOptions:
     soft_start: bool = False
 
     def to_struct(self, want_w: bool, padded: bool, warm: bool = False, wg_skip: int = 0) -> _lib.XtbScfOpts:
+        if not self.fermi_etemp >= 0.0:
+            # filling.py:320-324
+            raise ValueError(f"Electronic Temperature must be positive or None ({self.fermi_etemp}).")
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
         return _lib.XtbScfOpts(
             maxiter=self.maxiter, mix_guess=int(self.mix_guess), generations=self.generations,
@@ -443,6 +459,16 @@ class Pipeline:
                                        self.gamma.data_ptr(), ct.byref(self.state),
                                        self.dedcn.data_ptr(), self.grad.data_ptr(), s),
                    "xtb_forces")
+
+    def repulsion_gradient(self, stream=None) -> torch.Tensor:
+        """dE_rep/dR of the last run's positions, (nat_total, 3)."""
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        g = torch.empty_like(self.grad)
+        _lib.check(self.lib.xtb_repulsion_gradient(ct.byref(self.batch), ct.byref(self.dt.s),
+                                                   self.positions.data_ptr(), g.data_ptr(),
+                                                   ct.c_void_p(st.cuda_stream)),
+                   "xtb_repulsion_gradient")
+        return g.view(-1, 3)
 
     def param_grad(self, stream=None) -> torch.Tensor:
         """dE/dtheta of the last run's energies for the species tables'

@@ -56,6 +56,16 @@ class Request:
     status: int = 0
     iterations: int = 0
 
+    @property
+    def failed(self) -> bool:
+        """The result is unusable: the overlap matrix was not positive
+        definite (no SCF ran), the Fermi level or the final eigensolve did
+        not converge, or non-finite values appeared.  Energy and forces are
+        then NaN.  An SCF that stopped at maxiter still returns its last
+        result (status bit MAXITER), as upstream warns and returns."""
+        bad = _lib.MOL_OVERLAP | _lib.MOL_FERMI | _lib.MOL_NONFINITE | _lib.MOL_JACOBI
+        return bool(self.status & bad)
+
 
 @dataclass
 class _Flight:
@@ -279,6 +289,10 @@ class RaggedServer:
             r.iterations = int(rec[b, 5])
             if fo is not None:
                 r.forces = fo[a0[b]:a0[b] + len(r.numbers)]
+            if r.failed:
+                r.energy = float("nan")
+                if r.forces is not None:
+                    r.forces = np.full_like(r.forces, np.nan)
             r.done = now
             self.results[r.rid] = r
             self.stats.latencies.append(now - r.arrival)

@@ -70,7 +70,9 @@ extern "C" {
 #define XTB_MOL_MAXITER 1   /* SCF not converged within maxiter */
 #define XTB_MOL_FERMI 2     /* Fermi level failed to converge */
 #define XTB_MOL_NONFINITE 4 /* NaN/Inf met */
-#define XTB_MOL_JACOBI 8    /* eigensolver hit its sweep limit */
+#define XTB_MOL_JACOBI 8    /* the final step's eigensolve hit its sweep limit */
+#define XTB_MOL_OVERLAP 16  /* overlap matrix not positive definite: no SCF was run
+                               (upstream's Cholesky reduction raises there) */
 
 /* layout of the per-molecule result record */
 #define XTB_OUT_STRIDE 32
@@ -124,7 +126,9 @@ typedef struct xtb_batch {
     const int32_t* mol_vec0;
     const int64_t* mol_mat0;
     const int64_t* mol_gam0;
-    const double* mol_nel;   /* [2*nmol] alpha, beta electrons */
+    const double* mol_nel;   /* [2*nmol] alpha, beta electrons, not rounded: Fermi
+                                smearing rounds them, the aufbau filling used
+                                below kT = 3e-7 Hartree takes them as they are */
     /* per atom [nat_total] */
     const int32_t* atom_sp;  /* species index into xtb_params */
     const int32_t* atom_mol;
@@ -178,7 +182,8 @@ typedef struct xtb_scf_opts {
     int32_t jacobi_inner;    /* sweeps per 32x32 block-Jacobi subproblem */
     double x_atol;
     double x_atol_max;
-    double kt;               /* electronic temperature, Hartree */
+    double kt;               /* electronic temperature, Hartree; below 3e-7 (0.1 K)
+                                the orbitals are filled by the aufbau principle */
     double damp;
     double damp_init;
     double diag_offset;
@@ -280,6 +285,14 @@ int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par,
 
 int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
                       const double* positions, double* out, void* stream);
+
+/* Gradient of the repulsion energy alone, per atom [3*nat_total] (written,
+ * not added); replaces the autograd of Repulsion.get_energy
+ * (src/dxtb/_src/components/classicals/repulsion/base.py:337-406).  With
+ * xtb_forces it gives the gradient when the repulsion or the electronic
+ * energy is excluded from the total. */
+int32_t xtb_repulsion_gradient(const xtb_batch* batch, const xtb_params* par,
+                               const double* positions, double* grad, void* stream);
 
 /* Nuclear gradient dE/dR of E = E_scf + (-TS) + E_rep, per atom [3*nat_total].
  * Needs the converged P, W (state.T when want_w), the orbital potential,

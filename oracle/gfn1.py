@@ -29,7 +29,16 @@ AA2AU = 1.0 / 0.52917721090380  # bohr radius (CODATA 2018), Angstrom
 # the oracle reproduces the tblite SCF energies of H2, H2O, CH4 and LYS_xao in
 # test/test_scf/samples.py to <1e-10 Eh; CODATA 2018 misses by 4.4e-8 relative.
 EV2AU = 1.0 / 27.21138505
-KELVIN2AU = 3.166811563455546e-06  # k_B in Hartree/K (CODATA 2018)
+# Boltzmann constant in Hartree/K as xTB/tblite use it.  Pinned by the tblite
+# free energies of the H, C, N, O atoms and ions held in
+# test/test_scf/test_elements_gfn1.py (fractional p occupations, so -TS is
+# ~3.6e-3 Eh): with this value five of them agree to 2e-12 Eh and all to
+# 3e-9; CODATA 2018 (3.166811563455546e-06) misses all fractional cases by
+# 3.4e-9 Eh.
+KELVIN2AU = 3.166808578545117e-06
+# electronic temperatures below this (0.1 K) take aufbau occupations instead
+# of Fermi smearing (_src/scf/base.py:888-889)
+KT_AUFBAU = 3e-7
 
 # Pyykkoe & Atsumi (2009) single-bond covalent radii (Angstrom) as used by
 # DFT-D3; D3 scales them by 4/3 (tad_mctc COV_D3).
@@ -543,13 +552,32 @@ def fermi_occupation(nel_channel: float, emo: torch.Tensor, kt: float,
 
 
 def alpha_beta(nel: float, spin=None):
-    """filling.py:41-113."""
-    uhf = 0.0 if spin is None else float(spin)
+    """filling.py:41-113: alpha/beta electrons; a spin that does not fit the
+    electron count raises ValueError (:80-93)."""
     neln = round(nel)
+    if spin is not None:
+        uhf = float(spin)
+        if uhf > neln:
+            raise ValueError(f"Number of unpaired electrons ({uhf}) larger than "
+                             f"number of electrons ({nel}).")
+        if uhf % 2 != neln % 2:
+            raise ValueError(f"Odd (even) number of unpaired electrons ({uhf}) but even "
+                             f"(odd) number of electrons ({nel}) given.")
+    else:
+        uhf = 0.0
     nuhf = uhf if (uhf % 2) == (neln % 2) else (neln % 2)
     diff = min(nuhf, nel)
     nb = (nel - diff) / 2.0
     return nb + diff, nb
+
+
+def aufbau_occupation(norb: int, nel: float) -> torch.Tensor:
+    """Aufbau filling of one channel (filling.py:116-198): orbital i (from 1)
+    is full while i <= nel, the one after takes the fractional remainder."""
+    idx = torch.arange(1, norb + 1, dtype=DTYPE)
+    nel_t = torch.tensor(float(nel), dtype=DTYPE)
+    frac = torch.remainder(nel_t, -1.0) + 1.0
+    return torch.heaviside(torch.ceil(nel_t - idx), frac.expand(norb))
 
 
 # ---------------------------------------------------------------------------
@@ -644,12 +672,18 @@ class _SCF:
         F = self.H0 - 0.5 * self.S * (v[:, None] + v[None, :])  # base.py:817-859
         self.fock = F
         e, C = eigh_generalized(F, self.S)
-        occ = []
-        for nel in self.nel_ab:
-            occ.append(fermi_occupation(round(nel), e, self.kt,
-                                        self.opts["fermi_maxiter"],
-                                        self.opts["fermi_thresh"]))
-        self.occ = torch.stack(occ)
+        if self.kt < KT_AUFBAU:
+            # base.py:888-889: no smearing; the aufbau occupations of the
+            # (unrounded) alpha/beta counts set up by get_refocc stay
+            # (iterator.py:147-189)
+            self.occ = torch.stack([aufbau_occupation(e.numel(), nel) for nel in self.nel_ab])
+        else:
+            occ = []
+            for nel in self.nel_ab:
+                occ.append(fermi_occupation(round(nel), e, self.kt,
+                                            self.opts["fermi_maxiter"],
+                                            self.opts["fermi_thresh"]))
+            self.occ = torch.stack(occ)
         self.emo, self.C = e, C
         f = self.occ.sum(0)
         return (C * f) @ C.T  # scf/utils.py:28-52
@@ -694,6 +728,9 @@ def singlepoint(numbers, positions, chrg: float = 0.0, spin=None,
     n0 = _n0(bas)
     nel = float(n0.sum()) - float(chrg)
     na, nb = alpha_beta(nel, spin)
+    if o["fermi_etemp"] < 0:
+        # filling.py:320-324
+        raise ValueError(f"Electronic Temperature must be positive or None ({o['fermi_etemp']}).")
     kt = o["fermi_etemp"] * KELVIN2AU
     scf = _SCF(bas, par, S, H0, gam, hd, n0, (na, nb), kt, o)
     mixer = Anderson(o["damp"], o["damp_init"], o["generations"], o["damp_diagonal_offset"])

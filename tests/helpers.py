@@ -10,13 +10,15 @@ TIGHT = dict(x_atol=1e-11, x_atol_max=1e-11, maxiter=300)
 TIGHT_ORACLE = {"x_atol": 1e-11, "x_atol_max": 1e-11, "maxiter": 300}
 
 
-def run_native(mols, padded=False, forces=False, chrg=None, **opt):
-    """mols: list of (numbers, positions[bohr]); returns the Pipeline."""
+def run_native(mols, padded=False, forces=False, chrg=None, spin=None, **opt):
+    """mols: list of (numbers, positions[bohr]); chrg, spin: per molecule
+    (spin None: automatic, as upstream); returns the Pipeline."""
     dev = torch.device("cuda:0")
     zs = np.unique(np.concatenate([z for z, _ in mols]))
     tb = SpeciesTables(zs)
     lay = BatchLayout([np.asarray(z) for z, _ in mols], tb, padded=padded,
-                      chrg=np.zeros(len(mols)) if chrg is None else np.asarray(chrg, dtype=float))
+                      chrg=np.zeros(len(mols)) if chrg is None else np.asarray(chrg, dtype=float),
+                      spin=None if spin is None else np.asarray(spin, dtype=float))
     pipe = Pipeline(lay, DeviceTables(tb, dev), dev)
     pos = torch.tensor(np.concatenate([p for _, p in mols]), dtype=torch.float64, device=dev)
     o = ScfOptions(**{**TIGHT, **opt})

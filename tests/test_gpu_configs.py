@@ -112,3 +112,35 @@ def test_command_line_matches_the_calculator(capsys):
     assert cli.main([str(g / "H2O.coord"), str(g / "NO2.xyz"), "--forces"]) == 0
     text = capsys.readouterr().out
     assert text.count("total energy") == 2 and text.count("Forces") == 2
+
+
+def test_config4_stream_slice_vs_oracle():
+    """Configuration 4 (serving stream, seed 4): the first 300 requests plus
+    the largest chain and the largest water cluster of the first 2000 (about
+    200 atoms, n ~ 530), served through the ragged continuous-batching loop;
+    energies and forces of 8 seeded requests (the two largest included)
+    against the oracle: 1e-9 Eh, 1e-8 Eh/bohr."""
+    from dxtb_amd.engine import ScfOptions
+    from dxtb_amd.serving import RaggedServer, Request
+
+    stream = W.config4(seed=4, count=2000)
+    big_chain = max((i for i in range(len(stream)) if 8 not in stream[i][1]),
+                    key=lambda i: len(stream[i][1]))
+    big_water = max((i for i in range(len(stream)) if 8 in stream[i][1] and 6 not in stream[i][1]),
+                    key=lambda i: len(stream[i][1]))
+    ids = list(range(300)) + [big_chain, big_water]
+    print(f"case: config 4 slice, {len(ids)} requests, largest {len(stream[big_chain][1])} "
+          f"and {len(stream[big_water][1])} atoms", flush=True)
+    reqs = [Request(i, stream[i][1], stream[i][2]) for i in ids]
+    srv = RaggedServer("cuda:0", opts=ScfOptions(**{k: v for k, v in TIGHT_ORACLE.items()}))
+    srv.run_stream(reqs, realtime=False)
+    assert all(srv.results[i].status == 0 for i in ids)
+    rng = np.random.default_rng(41)
+    check = [big_chain, big_water] + [int(i) for i in rng.choice(300, 6, replace=False)]
+    for i in check:
+        z, x = stream[i][1], stream[i][2]
+        print(f"case: config 4 request {i}, {len(z)} atoms, oracle", flush=True)
+        r = og.singlepoint(z, x, opts=TIGHT_ORACLE, with_forces=True)
+        got = srv.results[i]
+        assert abs(got.energy - r["energy"]) < 1e-9, i
+        assert np.abs(got.forces + r["gradient"].numpy()).max() < 1e-8, i

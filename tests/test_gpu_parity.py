@@ -248,6 +248,15 @@ def test_warm_start_along_a_trajectory(fx):
             assert np.abs(qa - qb).max() < 1e-8
         print(f"geometry {g}: iterations warm {rw[:, 5]} cold {rc[:, 5]}, first-step sweeps "
               f"warm {rw[:, 18]} cold {rc[:, 18]}", flush=True)
+        # and against the oracle at every geometry (energies, charges, forces)
+        Ew, Gw = pipe.energies().cpu().numpy(), pipe.gradient().cpu().numpy()
+        for k, (z, p) in enumerate(mols):
+            r = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
+            assert abs(Ew[k] - r["energy"]) < 1e-9, (g, k)
+            qw = pipe.orbital_charges(k).cpu().numpy()
+            assert np.abs(qw - r["charges_orbital"].numpy()).max() < 1e-8, (g, k)
+            a0 = int(pipe.L.atom0[k])
+            assert np.abs(Gw[a0:a0 + len(z)] - r["gradient"].numpy()).max() < 1e-8, (g, k)
         if g > 0:
             # the saving is the first step's eigensolve (the kept orbitals
             # replace it); with random 0.03 bohr displacements the iteration

@@ -158,3 +158,64 @@ def test_calculator_parameter_gradient_names_and_units(fx):
         else:
             an = d["hamiltonian"][name[0]].item()
         assert abs(fd - an) < 1e-7 * max(1.0, abs(fd)), (name, fd, an)
+
+
+def test_parameter_gradient_vs_oracle_finite_differences():
+    """The calculator's dE/dtheta (names and units of the parameter file)
+    against central differences of the CPU oracle's energy in the same
+    parameter, one parameter of each class: a self-energy and its CN shift,
+    the distance polynomial, the shell hardness and the element's Hubbard
+    parameter, the third-order derivative, both repulsion parameters, a
+    Slater exponent, an electronegativity, and the Hamiltonian section's
+    shell, kpol, enscale and kpair scalings.  The oracle is independent of
+    the kernel (its own integrals and SCF), so a wrong chain rule or unit
+    in either fails here.  Relative tolerance 1e-6 (the differences carry
+    the SCF convergence noise, ~1e-12 Eh over 2h)."""
+    import copy
+
+    import oracle.gfn1 as og
+    from dxtb_amd import GFN1Calculator
+
+    rng = np.random.default_rng(31)
+    z, p = W.chain(rng, 5, p_n=0.4, p_o=0.4)
+    assert {7, 8} <= set(z.tolist())
+    dev = torch.device("cuda:0")
+    calc = GFN1Calculator(torch.tensor(z, device=dev), dtype=torch.float64,
+                          opts={"x_atol": 1e-11, "x_atol_max": 1e-11, "maxiter": 300, "exclude": ["disp"]})
+    print("case: parameter gradients vs oracle differences", flush=True)
+    d = calc.get_param_gradient(torch.tensor(p, dtype=torch.float64, device=dev))
+    par0 = og.load_params()
+    opts = {"x_atol": 1e-12, "x_atol_max": 1e-12, "maxiter": 300}
+    checks = [
+        (("elements", "C", "levels", 1), lambda: d["C"]["levels"][1]),
+        (("elements", "N", "kcn", 0), lambda: d["N"]["kcn"][0]),
+        (("elements", "O", "shpoly", 1), lambda: d["O"]["shpoly"][1]),
+        (("elements", "C", "lgam", 1), lambda: d["C"]["lgam"][1]),
+        (("elements", "H", "gam"), lambda: d["H"]["gam"]),
+        (("elements", "O", "gam3"), lambda: d["O"]["gam3"]),
+        (("elements", "N", "zeff"), lambda: d["N"]["zeff"]),
+        (("elements", "C", "arep"), lambda: d["C"]["arep"]),
+        (("elements", "H", "slater", 0), lambda: d["H"]["slater"][0]),
+        (("elements", "O", "en"), lambda: d["O"]["en"]),
+        (("hamiltonian", "shell", "sp"), lambda: d["hamiltonian"]["shell"]["sp"]),
+        (("hamiltonian", "kpol"), lambda: d["hamiltonian"]["kpol"]),
+        (("hamiltonian", "enscale"), lambda: d["hamiltonian"]["enscale"]),
+        (("hamiltonian", "kpair", "N-H"), lambda: d["hamiltonian"]["kpair"]["N-H"]),
+    ]
+    for path, get in checks:
+        node = par0
+        for k in path[:-1]:
+            node = node[k]
+        base = node[path[-1]]
+        h = 1e-4 * max(1.0, abs(base))
+        e = []
+        for s in (1.0, -1.0):
+            par = copy.deepcopy(par0)
+            node = par
+            for k in path[:-1]:
+                node = node[k]
+            node[path[-1]] = base + s * h
+            e.append(og.singlepoint(z, p, opts=opts, par=par)["energy"])
+        fd = (e[0] - e[1]) / (2 * h)
+        an = float(get())
+        assert abs(fd - an) < 1e-6 * max(abs(fd), 1e-2), (path, fd, an)

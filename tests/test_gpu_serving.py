@@ -96,3 +96,80 @@ def test_staged_results_equal_plain_copies():
         assert r.status == 0 and s.status == 0
         assert abs(r.energy - s.energy) < 1e-12
         assert np.abs(r.forces - s.forces).max() < 1e-12
+
+
+def test_overlap_not_positive_definite_raises():
+    """Twelve H atoms 0.102 bohr apart on a line: the overlap matrix has an
+    eigenvalue of -7e-10 in FP64, so upstream's Cholesky reduction raises.
+    The SCF kernel skips such a molecule (status bit OVERLAP) and the
+    calculator raises torch.linalg.LinAlgError; a serving request is marked
+    failed with NaN results.  A normal molecule in the same batch is
+    unaffected."""
+    dev = torch.device("cuda:0")
+    line = np.zeros((12, 3))
+    line[:, 2] = 0.102 * np.arange(12)
+    good = W.chain(np.random.default_rng(4), 3)
+    print("case: overlap not positive definite", flush=True)
+    pipe = run_native([(np.ones(12, dtype=np.int64), line), good], forces=True, **OPTS)
+    st = pipe.records().cpu().numpy()[:, 6].astype(int)
+    assert st[0] & 16 and st[1] == 0
+    calc = GFN1Calculator(torch.ones(12, dtype=torch.int64, device=dev),
+                          opts={**OPTS, "exclude": ["disp"]}, dtype=torch.float64)
+    with pytest.raises(torch.linalg.LinAlgError):
+        calc.get_energy(torch.tensor(line, device=dev))
+    srv = RaggedServer("cuda:0", opts=ScfOptions(**OPTS), streams=2)
+    reqs = [Request(0, np.ones(12, dtype=np.int64), line), Request(1, *good)]
+    srv.run_stream(reqs, realtime=False)
+    assert srv.results[0].failed and np.isnan(srv.results[0].energy)
+    assert np.all(np.isnan(srv.results[0].forces))
+    assert not srv.results[1].failed and np.isfinite(srv.results[1].energy)
+
+
+def test_excluded_components_with_autograd():
+    """opts["exclude"] with positions that require grad (upstream's autograd
+    covers any component selection): without the repulsion, without the
+    electronic energy, and without both, against the oracle's pieces."""
+    import oracle.gfn1 as og
+
+    dev = torch.device("cuda:0")
+    z, p = W.chain(np.random.default_rng(13), 6)
+    r = og.singlepoint(z, p, opts={"x_atol": 1e-10, "x_atol_max": 1e-10, "maxiter": 300},
+                       with_forces=True)
+    _, grep = og.repulsion(list(z), torch.tensor(p), grad=True)
+    want = {
+        "rep": (r["energy"] - r["repulsion"], r["gradient"] - grep),
+        "scf": (r["repulsion"], grep),
+        "all": (0.0, torch.zeros_like(grep)),
+    }
+    print("case: excluded components with autograd", flush=True)
+    for ex, (e_ref, g_ref) in want.items():
+        calc = GFN1Calculator(torch.tensor(z, device=dev), opts={**OPTS, "exclude": ["disp", ex]},
+                              dtype=torch.float64)
+        pos = torch.tensor(p, dtype=torch.float64, device=dev, requires_grad=True)
+        e = calc.get_energy(pos)
+        assert abs(e.item() - e_ref) < 1e-9, ex
+        f = calc.get_forces(pos)
+        assert (f.cpu() + g_ref).abs().max().item() < 1e-8, ex
+
+
+def test_result_cache_checks_positions_by_value():
+    """With cache_enabled a run is reused only for equal positions, charges
+    and spin: a new tensor that happens to reuse a freed tensor's storage
+    gets its own result.  Without it (upstream's default) nothing is
+    reused."""
+    dev = torch.device("cuda:0")
+    z, p = W.chain(np.random.default_rng(14), 5)
+    numbers = torch.tensor(z, device=dev)
+    print("case: result cache", flush=True)
+    for cache in (True, False):
+        calc = GFN1Calculator(numbers, opts={**OPTS, "exclude": ["disp"], "cache_enabled": cache},
+                              dtype=torch.float64)
+        fresh = GFN1Calculator(numbers, opts={**OPTS, "exclude": ["disp"]}, dtype=torch.float64)
+        for k in range(6):
+            q = p.copy()
+            q[0, 0] += 0.01 * k  # a distortion (a translation would not change E)
+            pos = torch.tensor(q, dtype=torch.float64, device=dev)
+            e = calc.get_energy(pos).item()
+            assert abs(e - fresh.get_energy(pos.clone()).item()) < 1e-12
+            assert abs(calc.get_energy(pos).item() - e) < 1e-12
+            del pos

@@ -192,3 +192,27 @@ def test_scf_rejects_an_out_of_range_xcd_shift():
     o = ScfOptions().to_struct(want_w=False, padded=False, wg_skip=8)
     rc = lib.xtb_scf(ct.byref(b), ct.byref(p), 8, 8, 8, ct.byref(o), ct.byref(st), None)
     assert rc == 1  # invalid argument
+
+
+def test_alpha_beta_rejects_spins_that_do_not_fit():
+    """filling.py:80-93: more unpaired electrons than electrons, or odd/even
+    parity mismatch, raise ValueError (they used to be replaced silently)."""
+    with pytest.raises(ValueError):
+        alpha_beta(np.array([8.0]), 1)
+    with pytest.raises(ValueError):
+        alpha_beta(np.array([3.0]), 5)
+    with pytest.raises(ValueError):
+        BatchLayout([np.array([8, 1, 1])], SpeciesTables([1, 8]), padded=False,
+                    chrg=np.zeros(1), spin=np.array([1.0]))
+    na, nb = alpha_beta(np.array([17.0, 8.0, 7.5]), None)
+    assert na.tolist() == [9.0, 4.0, 3.75] and nb.tolist() == [8.0, 4.0, 3.75]
+    na, nb = alpha_beta(np.array([8.0]), 2)
+    assert (na[0], nb[0]) == (5.0, 3.0)
+
+
+def test_negative_temperature_is_rejected():
+    from dxtb_amd.engine import ScfOptions
+
+    with pytest.raises(ValueError):
+        ScfOptions(fermi_etemp=-1.0).to_struct(want_w=False, padded=False)
+    ScfOptions(fermi_etemp=0.0).to_struct(want_w=False, padded=False)

@@ -129,6 +129,72 @@ def test_forces_central_difference(fx):
             assert abs(fd - float(r["gradient"][a, x])) < 1e-8
 
 
+def _free_energy(f, kt):
+    """-TS of occupations f (base.py:580-623 restated)."""
+    eps = torch.finfo(torch.float64).eps
+    o, o1 = f.clamp(min=eps), (1 - f).clamp(min=eps)
+    return float(torch.log(o ** o * o1 ** o1).sum() * kt)
+
+
+def test_fermi_matches_upstream_sih4_vectors(fx):
+    """Fermi smearing at 5000 K and aufbau filling at 0 K against the SiH4
+    vectors of test_wavefunction/test_filling.py:186-270.  The 5000 K
+    occupations are fractional down to 1e-19, so a wrong exponent, cutoff,
+    derivative or Boltzmann constant moves them by more than 1e-8."""
+    emo = torch.tensor(fx["sih4_emo"])
+    nel = float(fx["sih4_nel"])
+    na, nb = og.alpha_beta(nel, 0)
+    kt = 5000 * og.KELVIN2AU
+    f = torch.stack([og.fermi_occupation(round(na), emo, kt), og.fermi_occupation(round(nb), emo, kt)])
+    assert np.abs(f.sum(0).numpy() - fx["sih4_focc_5000"]).max() < 5e-9
+    assert abs(_free_energy(f, kt) - float(fx["sih4_fenergy_5000"])) < 1e-8
+    f0 = torch.stack([og.aufbau_occupation(emo.numel(), na), og.aufbau_occupation(emo.numel(), nb)])
+    assert np.array_equal(f0.sum(0).numpy(), fx["sih4_focc_0"])
+    assert _free_energy(f0, 0.0) == 0.0
+
+
+def test_aufbau_matches_upstream_doctest():
+    """filling.py:135-147: fractional electron counts fill the next orbital
+    with the remainder."""
+    for norb, nel, ref in ((5, 1.0, [1, 0, 0, 0, 0]), (8, 3.0, [1, 1, 1, 0, 0, 0, 0, 0]),
+                           (4, 3.5, [1, 1, 1, 0.5]), (2, 1.5, [1, 0.5])):
+        assert og.aufbau_occupation(norb, nel).tolist() == ref
+
+
+def test_atoms_and_ions_match_tblite(fx):
+    """Open shells, charges and fractional occupations: tblite free energies
+    (E_scf - TS at 300 K) of the H, C, N, O atoms and ions
+    (test_scf/test_elements_gfn1.py).  The degenerate p shells are partly
+    filled, so -TS is ~3.6e-3 Eh and the Boltzmann constant matters at 1e-9."""
+    opts = {"x_atol": 1e-12, "x_atol_max": 1e-12, "maxiter": 300, "fermi_thresh": 1e-10}
+    for z, c, s, ref in zip(fx["atoms_z"], fx["atoms_chrg"], fx["atoms_spin"], fx["atoms_escf"]):
+        r = og.singlepoint([int(z)], np.zeros((1, 3)), chrg=float(c), spin=None if s < 0 else int(s), opts=opts)
+        assert r["converged"]
+        assert abs(r["scf"] + r["fenergy"] - float(ref)) < 5e-9, (z, c)
+
+
+def test_charged_molecule_matches_tblite(fx):
+    """AD7en+ (C14H25+, charge +1): tblite SCF energy
+    (test_scf/samples_charged.py:57-61)."""
+    r = og.singlepoint(fx["ad7en_numbers"], fx["ad7en_positions"], chrg=float(fx["ad7en_chrg"]), opts=TIGHT)
+    assert r["converged"]
+    assert abs(r["scf"] - float(fx["ad7en_escf"])) < 1e-9
+
+
+def test_spin_that_does_not_fit_raises():
+    """filling.py:80-93: more unpaired electrons than electrons, or the wrong
+    parity, raise ValueError."""
+    with pytest.raises(ValueError):
+        og.alpha_beta(8.0, 1)
+    with pytest.raises(ValueError):
+        og.alpha_beta(3.0, 5)
+    assert og.alpha_beta(17.0, 1) == (9.0, 8.0)
+    assert og.alpha_beta(17.0, None) == (9.0, 8.0)
+    assert og.alpha_beta(8.0, 2) == (5.0, 3.0)
+    with pytest.raises(ValueError):
+        og.singlepoint([8, 1, 1], np.array([[0.0, 0, 0], [0, 0, 1.8], [1.8, 0, 0]]), opts={"fermi_etemp": -1.0})
+
+
 def test_fermi_closed_form():
     # two-level system with a tiny gap: occupations follow 1/(1+exp) exactly
     kt = 300 * og.KELVIN2AU
