@@ -5,20 +5,23 @@ chains, 30 % water clusters), energy + analytic forces in FP64, seed 4.  A
 "step" hands the next `--chunk` requests of the stream, all available at
 once, to the continuous-batching serving loop, which admits them as streams
 free up while the previous chunk's molecules finish (saturation throughput
-of the loop; the timed region ends when every chunk is done).  The JSON line
-also carries
+of the loop; the timed region ends when every chunk is done).  The K steps
+are timed `--repeats` times; the value is the median.  The JSON line also
+carries
 
 * the same chunks served one at a time, each drained before the next starts;
-
 * the plain padded-batch drop-in on the same requests (arrival order,
-  `--padded-batch` molecules per padded batch) and the ragged/padded ratio;
-* p50/p99 latency of the serving loop under Poisson arrivals at 70 % of the
-  measured saturation rate (the JSON's p50/p99), and at 50 %
-  (`latency_curve`);
+  `--padded-batch` molecules per padded batch: 64, and 256, which fills the
+  chip's 256 CUs) and the ragged/padded ratios;
+* p50/p99 latency of the serving loop on the whole 20000-request stream under
+  Poisson arrivals at 30/50/70/90 % of the measured saturation rate
+  (`latency_curve`; the headline p50/p99 at 70 %);
 * a roofline entry for the dominant kernel (the persistent SCF kernel,
   FP64): algorithmic flops / duration from HIP events of one launch over
-  as many requests as the serving loop holds (`--lookahead` x `--chunk`),
-  alone on the GPU;
+  requests 600-2999 (as many as the serving loop holds), alone on the GPU,
+  with its HBM traffic and executed matrix-core flops from the committed
+  rocprofv3 summaries of that launch, and the stage rates of the integral
+  pass and of the kernel's Fock-build and Mulliken stages;
 * the CPU baseline: the oracle (a PyTorch FP64 restatement of the upstream
   algorithm) timed on this host on a bounded prefix of the same stream.
 
@@ -206,13 +209,25 @@ def device_peaks(device) -> dict:
     return {"dgemm_tflops": round(gemm, 2), "copy_gbs": round(bw, 1)}
 
 
-def roofline_pass(stream, start: int, count: int, device, opts) -> dict:
+# the roofline pass's workload: requests 600-2999 of the seed-4 stream (as
+# many as the serving loop holds: 4 chunks of 600) in one launch.  Fixed, so
+# that the committed rocprofv3 summaries (traffic, matrix-core counters) are
+# of exactly the launch this run times, whatever --warmup or --chunk are.
+ROOFLINE_START, ROOFLINE_COUNT = 600, 2400
+HBM_PEAK_GBS = 8000.0
+NCU = 256
+
+
+def roofline_pass(stream, device, opts, start: int = ROOFLINE_START, count: int = ROOFLINE_COUNT) -> dict:
     """The SCF kernel alone in one launch: `count` requests of the stream
     (as many as the serving loop holds) in one ragged batch, one workgroup
     per molecule, largest first, so that the dispatcher fills CUs as they
     free up; one stream, HIP events around the k_scf launch on that stream,
     energy + forces settings; algorithmic flops from the measured iteration
-    counts."""
+    counts.  Also the integral pass (CN, S, H0, gamma) of the same batch
+    with its algorithmic bytes (16 n^2 for S and H0, 8 m^2 for gamma), and
+    the SCF kernel's Fock-build and Mulliken/potential stages from its
+    device timers (24 n^2 and 16 n^2 + 8 m^2 bytes per pass)."""
     import torch
 
     from dxtb_amd import _lib
@@ -233,6 +248,8 @@ def roofline_pass(stream, start: int, count: int, device, opts) -> dict:
     st.synchronize()
     ev0, ev1 = pipe.scf_events
     ms = float(ev0.elapsed_time(ev1))
+    ei0, ei1 = pipe.int_events
+    int_ms = float(ei0.elapsed_time(ei1))
     fl = pipe.scf_flops()
     rec = pipe.records().cpu().numpy()
     bad = int((rec[:, _lib.OUT_STATUS] != 0).sum())
@@ -240,20 +257,49 @@ def roofline_pass(stream, start: int, count: int, device, opts) -> dict:
     # the same flops over the workgroups' own device time (one workgroup per
     # molecule, one CU each): the kernel's efficiency while resident,
     # independent of how many CUs are busy
-    per_cu_peak = FP64_PEAK_TFLOPS * 1e12 / 256.0
+    per_cu_peak = FP64_PEAK_TFLOPS * 1e12 / NCU
     busy = float(rec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
     longest = float(rec[:, _lib.OUT_T_TOTAL].max()) * 1e-3
+    n2 = lay.n.astype(np.float64) ** 2
+    m2 = lay.nsh.astype(np.float64) ** 2
+    int_bytes = float(16.0 * n2.sum() + 8.0 * m2.sum())
+    # stages inside the SCF kernel: bytes per workgroup-microsecond; one
+    # workgroup owns one CU, whose share of the HBM peak is peak / 256
+    nf, tf = rec[:, _lib.OUT_N_FOCK], rec[:, _lib.OUT_T_FOCK]
+    nm, tmu = rec[:, _lib.OUT_N_MULL], rec[:, _lib.OUT_T_MULL]
+    fock_bytes = float((24.0 * nf * n2).sum())
+    mull_bytes = float((16.0 * nm * n2 + 8.0 * (nm - 0.5) * m2).sum())
+    fock_gbs = fock_bytes / (float(tf.sum()) * 1e-6) / 1e9
+    mull_gbs = mull_bytes / (float(tmu.sum()) * 1e-6) / 1e9
+    share = HBM_PEAK_GBS / NCU
+    stages = {
+        "integrals_s_h0_gamma": {"ms": round(int_ms, 3), "bytes": int_bytes,
+                                 "gbs": round(int_bytes / (int_ms * 1e-3) / 1e9, 1),
+                                 "frac_of_hbm_peak": round(int_bytes / (int_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                                 "bytes_rule": "16 n^2 (S, H0) + 8 m^2 (gamma) per molecule"},
+        "fock_build": {"passes": int(nf.sum()), "workgroup_s": round(float(tf.sum()) * 1e-6, 4),
+                       "gbs_per_workgroup": round(fock_gbs, 2),
+                       "frac_of_cu_share": round(fock_gbs / share, 4),
+                       "share_of_scf_time": round(float(tf.sum()) * 1e-6 / busy, 4),
+                       "bytes_rule": "24 n^2 per pass (read H0, S; write F)"},
+        "mulliken_potential": {"passes": int(round(float((nm - 0.5).sum()))),
+                               "workgroup_s": round(float(tmu.sum()) * 1e-6, 4),
+                               "gbs_per_workgroup": round(mull_gbs, 2),
+                               "frac_of_cu_share": round(mull_gbs / share, 4),
+                               "share_of_scf_time": round(float(tmu.sum()) * 1e-6 / busy, 4),
+                               "bytes_rule": "16 n^2 (P, S; +8 n^2 H0 in the last pass) + 8 m^2 (gamma)"},
+        "cu_share_gbs": share,
+    }
     return {"achieved": achieved, "launches": 1, "mean_launch_ms": ms, "flops_per_launch": fl,
             "molecules": count, "requests": f"{start}-{start + count - 1}", "nonconverged": bad,
-            "resident_frac": fl / (busy * per_cu_peak), "cu_occupancy": busy / (256.0 * ms * 1e-3),
-            "longest_molecule_ms": longest}
+            "resident_frac": fl / (busy * per_cu_peak), "cu_occupancy": busy / (NCU * ms * 1e-3),
+            "longest_molecule_ms": longest, "stages": stages}
 
 
-def measured_traffic(start: int, count: int) -> float | None:
-    """HBM bytes of the roofline pass's k_scf launch, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) summarised in
-    profiles/k_scf_traffic.json; None when the workload differs."""
-    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_scf_traffic.json")
+def _profile_json(name: str, start: int, count: int) -> dict | None:
+    """A committed rocprofv3 summary of the roofline pass's k_scf launch;
+    None when it is missing or of another workload."""
+    f = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", name)
     try:
         with open(f) as fh:
             t = json.load(fh)
@@ -261,7 +307,31 @@ def measured_traffic(start: int, count: int) -> float | None:
         return None
     if t.get("start") != start or t.get("count") != count or t.get("seed") != 4:
         return None
-    return float(t["traffic_bytes_per_launch"])
+    return t
+
+
+def measured_traffic(start: int = ROOFLINE_START, count: int = ROOFLINE_COUNT) -> float | None:
+    """HBM bytes of the roofline pass's k_scf launch, from the rocprofv3
+    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) summarised in
+    profiles/k_scf_traffic.json."""
+    t = _profile_json("k_scf_traffic.json", start, count)
+    return None if t is None else float(t["traffic_bytes_per_launch"])
+
+
+def measured_mfma(ms: float, start: int = ROOFLINE_START, count: int = ROOFLINE_COUNT) -> dict | None:
+    """Executed FP64 matrix-core work of the same launch from the rocprofv3
+    SQ_INSTS_VALU_MFMA_MOPS_F64 run (units of 512 flops) summarised in
+    profiles/k_scf_mfma.json, over this run's launch time: what the matrix
+    cores executed, next to the fixed-convention flops of `achieved`."""
+    t = _profile_json("k_scf_mfma.json", start, count)
+    if t is None:
+        return None
+    fl = float(t["mfma_f64_flops_per_launch"])
+    out = {"executed_flops_per_launch": fl, "executed_tflops": round(fl / (ms * 1e-3) / 1e12, 3),
+           "executed_frac": round(fl / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS, 4)}
+    if "mfma_busy_frac" in t:
+        out["mfma_busy_frac"] = t["mfma_busy_frac"]
+    return out
 
 
 def main() -> int:
@@ -274,8 +344,14 @@ def main() -> int:
                     help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-16)")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--chunk", type=int, default=600)
-    ap.add_argument("--latency-requests", type=int, default=2000)
-    ap.add_argument("--padded-batch", type=int, default=64)
+    ap.add_argument("--latency-requests", type=int, default=20000,
+                    help="requests of the stream served under Poisson arrivals per rate")
+    ap.add_argument("--latency-rates", default="0.3,0.5,0.7,0.9",
+                    help="arrival rates as fractions of the measured saturation rate")
+    ap.add_argument("--repeats", type=int, default=3,
+                    help="timed repetitions of the K steps (value = median)")
+    ap.add_argument("--padded-batch", default="64,256",
+                    help="molecules per padded batch of the drop-in (first = headline ratio)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-padded", action="store_true")
@@ -323,14 +399,14 @@ def main() -> int:
                           "bytes_written": y.numel() * 8}))
         return 0
     opts = ScfOptions(x_atol=1e-10, x_atol_max=1e-10, maxiter=300)
-    total = (a.warmup + a.steps) * a.chunk + a.latency_requests
+    total = max((a.warmup + a.steps) * a.chunk, a.latency_requests, ROOFLINE_START + ROOFLINE_COUNT)
     stream = W.config4(seed=4, count=total)
     log(f"stream: {total} requests generated")
 
     if a.profile_run:
         # the roofline pass alone (for rocprofv3 runs): its mean launch time is
         # the figure the kernel trace's average duration of k_scf must match
-        rp = roofline_pass(stream, a.warmup * a.chunk, a.lookahead * a.chunk, dev, opts)
+        rp = roofline_pass(stream, dev, opts)
         print(json.dumps({"profile_run": rp}))
         return 0
     server = RaggedServer(dev, opts=opts)
@@ -366,30 +442,39 @@ def main() -> int:
     # continuous batching: step k hands chunk k to the loop, which admits it
     # as streams free up while earlier chunks finish; once `lookahead` chunks
     # are held, the step ends when the oldest of them is done.  The timed
-    # region ends when every chunk is done.
-    torch.cuda.synchronize()
-    busy0 = server.stats.scf_busy_s
-    t0 = time.perf_counter()
-    served, held = [], []
-    for k in range(a.steps):
-        reqs = _requests(stream, pos, a.chunk)
-        server.feed(reqs)
-        held.append(reqs)
-        if len(held) >= a.lookahead:
-            server.serve_until(held.pop(0))
-        served += reqs
-        pos += a.chunk
-        log(f"step {k} handed over")
-    server.serve_until(served)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    bad = sum(1 for r in served if server.results[r.rid].status != 0)
-    value = a.steps * a.chunk / (t1 - t0)
-    ms_step = (t1 - t0) * 1e3 / a.steps
-    # share of the CU-time the SCF kernel's workgroups held (one CU each)
+    # region ends when every chunk is done.  Repeated `--repeats` times on
+    # the same chunks; the value is the median.
     ncu = torch.cuda.get_device_properties(dev).multi_processor_count
-    occupancy = (server.stats.scf_busy_s - busy0) / (ncu * (t1 - t0))
-    log(f"ragged: {value:.1f} molecules/s")
+    runs = []
+    for rep in range(a.repeats):
+        torch.cuda.synchronize()
+        busy0 = server.stats.scf_busy_s
+        pos = a.warmup * a.chunk
+        t0 = time.perf_counter()
+        served, held = [], []
+        for k in range(a.steps):
+            reqs = _requests(stream, pos, a.chunk)
+            server.feed(reqs)
+            held.append(reqs)
+            if len(held) >= a.lookahead:
+                server.serve_until(held.pop(0))
+            served += reqs
+            pos += a.chunk
+            if rep == 0:
+                log(f"step {k} handed over")
+        server.serve_until(served)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        bad = sum(1 for r in served if server.results[r.rid].status != 0)
+        # share of the CU-time the SCF kernel's workgroups held (one CU each)
+        occ_r = (server.stats.scf_busy_s - busy0) / (ncu * (t1 - t0))
+        runs.append((a.steps * a.chunk / (t1 - t0), (t1 - t0) * 1e3 / a.steps, occ_r, bad))
+        log(f"repeat {rep}: ragged {runs[-1][0]:.1f} molecules/s")
+        server.results.clear()
+    order = sorted(range(len(runs)), key=lambda i: runs[i][0])
+    value, ms_step, occupancy, bad = runs[order[len(runs) // 2]]
+    bad = max(r[3] for r in runs)
+    log(f"ragged: {value:.1f} molecules/s (median of {len(runs)})")
     # the same chunks one at a time, each drained before the next starts
     barrier = None
     if not a.no_barrier:
@@ -401,29 +486,28 @@ def main() -> int:
             bsrv.run_stream(_requests(stream, (a.warmup + k) * a.chunk, a.chunk), realtime=False)
         torch.cuda.synchronize()
         barrier = a.steps * a.chunk / (time.perf_counter() - tb0)
+        bsrv.results.clear()
         log(f"ragged, chunks drained one at a time: {barrier:.1f} molecules/s")
 
     # roofline of the SCF kernel: one launch over the requests the loop holds
-    rp = roofline_pass(stream, a.warmup * a.chunk, a.lookahead * a.chunk, dev, opts)
+    rp = roofline_pass(stream, dev, opts)
     achieved = rp["achieved"]
     log(f"scf kernel: one launch of {rp['molecules']} molecules, {rp['mean_launch_ms']:.2f} ms, "
         f"{achieved:.3f} TFLOP/s, CU occupancy {rp['cu_occupancy']:.3f}, "
         f"longest molecule {rp['longest_molecule_ms']:.1f} ms")
+    log(f"stages: {json.dumps(rp['stages'])}")
 
-    # latency under Poisson arrivals at 70 % of the saturation rate (and at
-    # 50 %, for the shape of the curve): the serving loop on the same process
-    # streams, with the results of the runs above released and the heap
-    # frozen again, as after start-up
-    server.results.clear()
+    # latency under Poisson arrivals at fractions of the saturation rate: the
+    # first `--latency-requests` requests of the stream (20000: the whole
+    # serving-stream configuration), the serving loop on the same process
+    # streams, the heap frozen again as after start-up
     served = held = None
-    if barrier is not None:
-        bsrv.results.clear()
     gc.collect()
     gc.freeze()
     curve = []
-    for frac in (0.5, 0.7):
+    for frac in [float(x) for x in a.latency_rates.split(",")]:
         r_ = frac * value
-        lat_reqs = _requests(stream, (a.warmup + a.steps) * a.chunk, a.latency_requests)
+        lat_reqs = _requests(stream, 0, a.latency_requests)
         rng = np.random.default_rng(44)
         t = 0.0
         for r in lat_reqs:
@@ -434,16 +518,20 @@ def main() -> int:
         log(f"latency @ {r_:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms "
             f"({lat['launches']} launches, host admission {ls.stats.host_admit_s:.3f} s, "
             f"{lat['seconds']:.3f} s for {len(lat_reqs)} requests)")
-        curve.append({"rate_per_s": round(r_, 1), "p50_ms": round(lat["p50_ms"], 2),
-                      "p99_ms": round(lat["p99_ms"], 2)})
+        curve.append({"rate_frac": frac, "rate_per_s": round(r_, 1), "served_per_s": round(lat["molecules_per_s"], 1),
+                      "p50_ms": round(lat["p50_ms"], 2), "p99_ms": round(lat["p99_ms"], 2)})
         ls.results.clear()
-    rate = curve[-1]["rate_per_s"]
+        del lat_reqs, ls
+        gc.collect()
+    head = min(curve, key=lambda c: abs(c["rate_frac"] - 0.7))
 
-    padded = None
+    padded = {}
     if not a.no_padded:
         pr = _requests(stream, a.warmup * a.chunk, a.chunk)
-        padded = padded_dropin(pr, a.padded_batch, dev, opts=opts)["molecules_per_s"]
-        log(f"padded drop-in: {padded:.1f} molecules/s")
+        for pb in [int(x) for x in a.padded_batch.split(",")]:
+            padded[pb] = padded_dropin(pr, pb, dev, opts=opts)["molecules_per_s"]
+            log(f"padded drop-in, batches of {pb}: {padded[pb]:.1f} molecules/s")
+    pb0 = int(a.padded_batch.split(",")[0])
 
     fc = forces_cost(_requests(stream, a.warmup * a.chunk, min(a.chunk, 128)), dev, opts)
     log(f"forces cost: {fc}")
@@ -468,18 +556,25 @@ def main() -> int:
             "batching": f"continuous: step k hands its chunk to the serving loop, which holds "
                         f"at most {a.lookahead} chunks and admits each as streams free up",
             "streams": server.streams.__len__(), "hw_queues": int(os.environ["GPU_MAX_HW_QUEUES"]),
+            "repeats_molecules_per_s": [round(r[0], 1) for r in runs],
             "scf_cu_occupancy": round(occupancy, 3),
             "chunk_barrier_molecules_per_s": None if barrier is None else round(barrier, 3),
-            "padded_dropin_molecules_per_s": None if padded is None else round(padded, 3),
-            "ragged_over_padded": None if padded is None else round(value / padded, 3),
-            "padded_batch": a.padded_batch,
-            "latency_rate_per_s": round(rate, 2), "p50_ms": curve[-1]["p50_ms"],
-            "p99_ms": curve[-1]["p99_ms"], "latency_curve": curve, "nonconverged": bad,
+            "padded_dropin_molecules_per_s": padded.get(pb0),
+            "ragged_over_padded": round(value / padded[pb0], 3) if pb0 in padded else None,
+            "padded_batch": pb0,
+            "padded_dropin_by_batch": {str(k): round(v, 2) for k, v in padded.items()},
+            "ragged_over_padded_by_batch": {str(k): round(value / v, 2) for k, v in padded.items()},
+            "padded_dropin_eigen": "padded layout: every step a full Jacobi eigensolve of the batch-maximum "
+                                   "n (no occupation-class or perturbative steps, as those need each "
+                                   "molecule's own gap); warm-started from the previous step's orbitals",
+            "latency_requests": a.latency_requests,
+            "latency_rate_per_s": head["rate_per_s"], "p50_ms": head["p50_ms"],
+            "p99_ms": head["p99_ms"], "latency_curve": curve, "nonconverged": bad,
             "forces_over_energy": fc, "md_warm_start": md,
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                     "traffic": measured_traffic(a.warmup * a.chunk, a.lookahead * a.chunk),
+                     "traffic": measured_traffic(),
                      "kernel": "xtb::k_scf",
                      "workload": f"one launch: requests {rp['requests']} of the stream, largest first",
                      "launches": rp["launches"], "mean_launch_ms": round(rp["mean_launch_ms"], 3),
@@ -487,6 +582,8 @@ def main() -> int:
                      "resident_frac": round(rp["resident_frac"], 4),
                      "cu_occupancy": round(rp["cu_occupancy"], 4),
                      "longest_molecule_ms": round(rp["longest_molecule_ms"], 2),
+                     "matrix_cores": measured_mfma(rp["mean_launch_ms"]),
+                     "stages": rp["stages"],
                      "measured_peaks": peaks},
         "cpu_baseline": cpu,
     }

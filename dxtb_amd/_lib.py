@@ -16,6 +16,9 @@ LIB_NAME = "libxtbhip.so"
 LIB_PATH = Path(__file__).resolve().parent / LIB_NAME
 
 XTB_OK = 0
+XTB_EINVAL, XTB_EHIP, XTB_ETOOBIG = 1, 2, 3
+# atoms a batch may hold in xtb_forces_noalloc (XTB_NOALLOC_MAX_ATOMS)
+NOALLOC_MAX_ATOMS = 262144
 STATUS_NAMES = {1: "invalid argument", 2: "HIP runtime error", 3: "size exceeds kernel limit"}
 
 MOL_MAXITER = 1
@@ -30,6 +33,7 @@ OUT_ITER, OUT_STATUS, OUT_EF_A, OUT_EF_B, OUT_EREP, OUT_SWEEPS = 5, 6, 7, 8, 9, 
 OUT_T_TOTAL = 14  # SCF kernel device time of the molecule, microseconds
 OUT_T_START = 20  # device wall clock at the molecule's start, microseconds
 OUT_XCD = 22  # XCD the molecule's workgroup ran on
+OUT_T_FOCK, OUT_N_FOCK, OUT_T_MULL, OUT_N_MULL = 24, 25, 26, 27  # stage timers (us) and counts
 
 P = ct.c_void_p
 i32 = ct.c_int32
@@ -120,6 +124,8 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_repulsion_gradient.argtypes = [bp, pp, P, P, P]
     lib.xtb_forces.restype = i32
     lib.xtb_forces.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P, P]
+    lib.xtb_forces_noalloc.restype = i32
+    lib.xtb_forces_noalloc.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P]
     lib.xtb_param_grad.restype = i32
     lib.xtb_param_grad.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfState), P, P]
     lib.xtb_param_grad_basis.restype = i32
@@ -140,6 +146,7 @@ EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
     "xtb_param_grad_basis", "xtb_build_s_h0_gamma", "xtb_repulsion_gradient",
+    "xtb_forces_noalloc",
 )
 
 

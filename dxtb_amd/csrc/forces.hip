@@ -17,6 +17,8 @@
 //  classicals/repulsion/base.py:337-406, src/dxtb/_src/ncoord/utils.py:30-52).
 #include "common.h"
 
+#include <mutex>
+
 namespace xtb {
 
 // zero two arrays in one small launch: every workgroup of a launch waits for
@@ -172,15 +174,10 @@ __global__ void __launch_bounds__(256) k_grad_cn(xtb_batch b, xtb_params par,
 
 using namespace xtb;
 
-extern "C" int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
-                              const double* positions, const double* cn, const double* gamma,
-                              const xtb_scf_state* state, double* dedcn, double* grad,
-                              void* stream) {
-    if (!batch || !par || !positions || !cn || !gamma || !state || !dedcn || !grad)
-        return XTB_EINVAL;
-    if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
-    if (!state->P || !state->T || !state->pot || !state->qsh) return XTB_EINVAL;
-    if (par->nus > XTB_MAX_US_LDS_F) return XTB_ETOOBIG;
+static int32_t launch_forces(const xtb_batch* batch, const xtb_params* par,
+                             const double* positions, const double* cn, const double* gamma,
+                             const xtb_scf_state* state, double* dedcn, double* grad,
+                             void* stream) {
     hipStream_t s = (hipStream_t)stream;
     hipLaunchKernelGGL(k_zero2, dim3(1), dim3(1024), 0, s, grad, (int64_t)3 * batch->nat_total, dedcn,
                        (int64_t)batch->nat_total);
@@ -191,4 +188,69 @@ extern "C" int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
                            positions, dedcn, grad);
     }
     return hipGetLastError() == hipSuccess ? XTB_OK : XTB_EHIP;
+}
+
+static int32_t check_forces_args(const xtb_batch* batch, const xtb_params* par,
+                                 const double* positions, const double* cn, const double* gamma,
+                                 const xtb_scf_state* state, const double* grad) {
+    if (!batch || !par || !positions || !cn || !gamma || !state || !grad) return XTB_EINVAL;
+    if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
+    if (!state->P || !state->T || !state->pot || !state->qsh) return XTB_EINVAL;
+    if (par->nus > XTB_MAX_US_LDS_F) return XTB_ETOOBIG;
+    return XTB_OK;
+}
+
+extern "C" int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
+                              const double* positions, const double* cn, const double* gamma,
+                              const xtb_scf_state* state, double* dedcn, double* grad,
+                              void* stream) {
+    const int32_t rc = check_forces_args(batch, par, positions, cn, gamma, state, grad);
+    if (rc != XTB_OK) return rc;
+    if (!dedcn) return XTB_EINVAL;
+    return launch_forces(batch, par, positions, cn, gamma, state, dedcn, grad, stream);
+}
+
+// ---------------------------------------------------------------------------
+// xtb_forces without the caller's scratch argument, for callers that cannot
+// allocate device memory.  dE/dCN (one double per atom, accumulated with
+// atomics by k_grad_pairs and read by k_grad_cn) lives in a file-scope
+// device array; its device address is looked up once per device with
+// hipGetSymbolAddress (the array's name in host code is only a host-side
+// stand-in, never a device address).  Every call on one device shares the
+// array: calls that overlap on one device race.  k_zero2 clears it on the
+// caller's stream before each use.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[XTB_NOALLOC_MAX_ATOMS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+}  // namespace
+
+static int32_t scratch_for_current_device(double** out) {
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess) return XTB_EHIP;
+    if (dev < 0 || dev >= kMaxDevices) return XTB_ETOOBIG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return XTB_EHIP;
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return XTB_OK;
+}
+
+extern "C" int32_t xtb_forces_noalloc(const xtb_batch* batch, const xtb_params* par,
+                                      const double* positions, const double* cn,
+                                      const double* gamma, const xtb_scf_state* state,
+                                      double* grad, void* stream) {
+    int32_t rc = check_forces_args(batch, par, positions, cn, gamma, state, grad);
+    if (rc != XTB_OK) return rc;
+    if (batch->nat_total > XTB_NOALLOC_MAX_ATOMS) return XTB_ETOOBIG;
+    double* dedcn = nullptr;
+    rc = scratch_for_current_device(&dedcn);
+    if (rc != XTB_OK) return rc;
+    return launch_forces(batch, par, positions, cn, gamma, state, dedcn, grad, stream);
 }

@@ -696,8 +696,14 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     const size_t nn = (size_t)n * n;
     // Fock build (scf/base.py:817-859)
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    fock_build(M, vin);
     double t0 = now_us();
+    fock_build(M, vin);
+    {
+        const double t = now_us();
+        sh.tacc[8] += t - t0;
+        sh.tacc[9] += 1.0;
+        t0 = t;
+    }
     int sw = 0;
     bool pd = false, intra = false;
     if (final_step) {
@@ -766,6 +772,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
             for (int c = lane; c < n; c += WAVE) M.Ct[(size_t)r * n + c] = M.F[(size_t)r * n + c];
     }
     sh.tacc[1] += now_us() - t0;
+    const double tm0 = now_us();
     // Mulliken charges and orbital energies (scf/base.py:764-815), one wave
     // per row; the orbital energies only enter the final energy (last step)
     {
@@ -832,6 +839,8 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         vout[i] = (s >= 0) ? sh.vsh[s] : 0.0;
     }
     __syncthreads();
+    sh.tacc[10] += now_us() - tm0;
+    sh.tacc[11] += last ? 1.5 : 1.0;  // the last step also reads H0 (orbital energies)
     return sw < 0 ? 0 : sw;
 }
 
@@ -1027,7 +1036,10 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         return;
     }
     int status = 0;
-    double tacc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    // stage timers and counters: 0 orthogonaliser, 1 GEMMs, 2 eigen, 3 PD,
+    // 4/5 block-Jacobi parts, 6 PD steps, 7 PD refresh count, 8/9 Fock build
+    // time/calls, 10/11 Mulliken + potential time/(P, S)-read units
+    double tacc[12] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     sh.tacc = tacc;
 #ifdef XTB_PD_TIMERS
     double ptacc[10] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
@@ -1214,6 +1226,7 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_XCD] = (double)(__builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 15);
         out[XTB_OUT_HW_ID] = (double)(unsigned)__builtin_amdgcn_s_getreg(4 | (0 << 6) | (31 << 11));
 #ifdef XTB_PD_TIMERS
+        // instrumented builds only: these overwrite slots 22-31
         for (int i = 0; i < 10; ++i) out[22 + i] = ptacc[i];
 #endif
         out[XTB_OUT_T_BSUB] = tacc[4];
@@ -1221,6 +1234,10 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         out[XTB_OUT_PD_STEPS] = tacc[6];
         out[XTB_OUT_SWEEPS_FIRST] = (double)sw_first;
         out[XTB_OUT_SWEEPS_LAST] = (double)sw_last;
+        out[XTB_OUT_T_FOCK] = tacc[8];
+        out[XTB_OUT_N_FOCK] = tacc[9];
+        out[XTB_OUT_T_MULL] = tacc[10];
+        out[XTB_OUT_N_MULL] = tacc[11];
     }
 }
 

@@ -101,6 +101,11 @@ extern "C" {
 #define XTB_OUT_T_PD 21     /* perturbative decoupling steps (part of T_EIGEN) */
 #define XTB_OUT_XCD 22      /* accelerator complex die (XCD) the molecule's workgroup ran on */
 #define XTB_OUT_HW_ID 23    /* the workgroup's HW_ID register (CU, shader array, engine) */
+#define XTB_OUT_T_FOCK 24   /* Fock builds F = H0 - S o (v_i + v_j)/2, microseconds */
+#define XTB_OUT_N_FOCK 25   /* Fock builds done (each reads H0 and S, writes F) */
+#define XTB_OUT_T_MULL 26   /* Mulliken charges + ES2/ES3 potential, microseconds */
+#define XTB_OUT_N_MULL 27   /* P and S reads of those passes (1.5 for the last:
+                               it also reads H0 for the orbital energies) */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {
@@ -301,6 +306,24 @@ int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
                    const double* positions, const double* cn,
                    const double* gamma, const xtb_scf_state* state,
                    double* dedcn, double* grad, void* stream);
+
+/* Atoms a batch may hold in xtb_forces_noalloc (its scratch: 8 bytes per
+ * atom, 2 MiB per device, held until the process ends). */
+#define XTB_NOALLOC_MAX_ATOMS 262144
+
+/* xtb_forces without the `dedcn` scratch argument: allocates nothing and
+ * takes no workspace; the scratch lives in a fixed device array of
+ * XTB_NOALLOC_MAX_ATOMS doubles inside the library, one per device.
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, results and status codes as
+ * xtb_forces; XTB_ETOOBIG when nat_total > XTB_NOALLOC_MAX_ATOMS.  All calls
+ * on one device share the scratch array: calls that overlap in time on one
+ * device (different streams or threads) race. */
+int32_t xtb_forces_noalloc(const xtb_batch* batch, const xtb_params* par,
+                           const double* positions, const double* cn,
+                           const double* gamma, const xtb_scf_state* state,
+                           double* grad, void* stream);
 
 /* Parameter gradient dE/dtheta of E = E_scf + (-TS) + E_rep at the converged
  * density (the energy is stationary in it, so no response term); replaces
