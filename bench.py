@@ -273,10 +273,10 @@ def roofline_pass(stream, device, opts, start: int = ROOFLINE_START, count: int 
     mull_gbs = mull_bytes / (float(tmu.sum()) * 1e-6) / 1e9
     share = HBM_PEAK_GBS / NCU
     stages = {
-        "integrals_s_h0_gamma": {"ms": round(int_ms, 3), "bytes": int_bytes,
+        "integral_pass": {"ms": round(int_ms, 3), "bytes": int_bytes,
                                  "gbs": round(int_bytes / (int_ms * 1e-3) / 1e9, 1),
                                  "frac_of_hbm_peak": round(int_bytes / (int_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
-                                 "bytes_rule": "16 n^2 (S, H0) + 8 m^2 (gamma) per molecule"},
+                                 "bytes_rule": "16 n^2 (S, H0) + 8 m^2 (gamma) per molecule; CN and repulsion in the same launch"},
         "fock_build": {"passes": int(nf.sum()), "workgroup_s": round(float(tf.sum()) * 1e-6, 4),
                        "gbs_per_workgroup": round(fock_gbs, 2),
                        "frac_of_cu_share": round(fock_gbs / share, 4),

@@ -120,6 +120,8 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_scf_lds_bytes.argtypes = [bp, ct.POINTER(i32), ct.POINTER(i32)]
     lib.xtb_repulsion.restype = i32
     lib.xtb_repulsion.argtypes = [bp, pp, P, P, P]
+    lib.xtb_build_integrals.restype = i32
+    lib.xtb_build_integrals.argtypes = [bp, pp, P, P, P, P, P, P, P]
     lib.xtb_repulsion_gradient.restype = i32
     lib.xtb_repulsion_gradient.argtypes = [bp, pp, P, P, P]
     lib.xtb_forces.restype = i32
@@ -146,7 +148,7 @@ EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
     "xtb_param_grad_basis", "xtb_build_s_h0_gamma", "xtb_repulsion_gradient",
-    "xtb_forces_noalloc",
+    "xtb_forces_noalloc", "xtb_build_integrals",
 )
 
 

@@ -15,18 +15,22 @@ namespace xtb {
 // ---------------------------------------------------------------------------
 // coordination number: D3 counting with exp_count,
 // CN_A = sum_{B != A, R <= cutoff} 1 / (1 + exp(-k (Rcov_AB / R - 1)))
-// (tad_mctc cn_d3 as called at src/dxtb/_src/xtb/gfn1.py:57-64)
+// (tad_mctc cn_d3 as called at src/dxtb/_src/xtb/gfn1.py:57-64).
+// CN_PARTS consecutive lanes share one atom: lane `part` sums the partners
+// B = atom0 + part, atom0 + part + CN_PARTS, ... in order, and a fixed
+// butterfly adds the parts.  Every kernel that needs CN uses this same
+// order, so all of them get the same bits.
 // ---------------------------------------------------------------------------
-__global__ void k_cn(xtb_batch b, xtb_params par, const double* __restrict__ pos,
-                     double* __restrict__ cn) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a >= b.nat_total) return;
+constexpr int CN_PARTS = 8;
+
+__device__ inline double cn_part(const xtb_batch& b, const xtb_params& par,
+                                 const double* __restrict__ pos, int a, int part) {
     const int m = b.atom_mol[a];
     const int a0 = b.mol_atom0[m], na = b.mol_nat[m];
     const double ra = par.sp_rcov[b.atom_sp[a]];
     const double x = pos[3 * a], y = pos[3 * a + 1], z = pos[3 * a + 2];
     double s = 0.0;
-    for (int j = a0; j < a0 + na; ++j) {
+    for (int j = a0 + part; j < a0 + na; j += CN_PARTS) {
         if (j == a) continue;
         const double dx = x - pos[3 * j], dy = y - pos[3 * j + 1], dz = z - pos[3 * j + 2];
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
@@ -34,7 +38,24 @@ __global__ void k_cn(xtb_batch b, xtb_params par, const double* __restrict__ pos
         const double rc = ra + par.sp_rcov[b.atom_sp[j]];
         s += 1.0 / (1.0 + exp(-par.cn_steep * (rc / r - 1.0)));
     }
-    cn[a] = s;
+    return s;
+}
+
+// sum over the CN_PARTS lanes of a group (aligned to CN_PARTS); the same
+// bits in every lane of the group (addition is commutative)
+__device__ inline double cn_combine(double v) {
+#pragma unroll
+    for (int o = 1; o < CN_PARTS; o <<= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_cn(xtb_batch b, xtb_params par, const double* __restrict__ pos,
+                                            double* __restrict__ cn) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = t / CN_PARTS, part = t % CN_PARTS;
+    const bool ok = a < b.nat_total;  // the butterfly needs every lane of the group
+    const double s = cn_combine(ok ? cn_part(b, par, pos, a, part) : 0.0);
+    if (ok && part == 0) cn[a] = s;
 }
 
 __device__ inline void stage_prims(const xtb_params& par, double* s_alpha, double* s_coeff) {
@@ -78,20 +99,10 @@ __device__ inline void gamma_pair(const xtb_batch& b, const xtb_params& par,
 // On-site blocks: S = identity on the shell's own orbitals, H0 = h on the
 // diagonal, zero between different shells of the same atom.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
-                                              const double* __restrict__ pos,
-                                              const double* __restrict__ cn,
-                                              double* __restrict__ S, double* __restrict__ H0,
-                                              double* __restrict__ gam) {
-    __shared__ double s_alpha[XTB_MAX_US_LDS * XTB_MAXPRIM];
-    __shared__ double s_coeff[XTB_MAX_US_LDS * XTB_MAXPRIM];
-    stage_prims(par, s_alpha, s_coeff);
-    const int* tl = b.tile_sh + 3 * blockIdx.x;
-    const int m = tl[0];
-    const int i = tl[1] + threadIdx.x / 16;
-    const int j = tl[2] + threadIdx.x % 16;
-    const int nsh = b.mol_nsh[m];
-    if (i >= nsh || j >= nsh || j < i) return;
+__device__ inline void shell_pair_s_h0(const xtb_batch& b, const xtb_params& par,
+                                       const double* __restrict__ pos, double* __restrict__ S,
+                                       double* __restrict__ H0, int m, int i, int j, double cni,
+                                       double cnj, const double* s_alpha, const double* s_coeff) {
     const int n = b.mol_n[m];
     const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
     const size_t mat0 = (size_t)b.mol_mat0[m];
@@ -100,8 +111,8 @@ __global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
     const int li = par.us_l[usi], lj = par.us_l[usj];
     const int ni = 2 * li + 1, nj = 2 * lj + 1;
     const int oi = b.sh_ao[sh0 + i], oj = b.sh_ao[sh0 + j];
-    const double hi = par.us_level[usi] - par.us_kcn[usi] * cn[at0 + ati];
-    const double hj = par.us_level[usj] - par.us_kcn[usj] * cn[at0 + atj];
+    const double hi = par.us_level[usi] - par.us_kcn[usi] * cni;
+    const double hj = par.us_level[usj] - par.us_kcn[usj] * cnj;
     double blk[3][3], hblk[3][3];
     for (int x = 0; x < 3; ++x)
         for (int y = 0; y < 3; ++y) { blk[x][y] = 0.0; hblk[x][y] = 0.0; }
@@ -136,6 +147,25 @@ __global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
             S[e2] = blk[x][y];
             H0[e2] = hblk[x][y];
         }
+}
+
+__global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
+                                              const double* __restrict__ pos,
+                                              const double* __restrict__ cn,
+                                              double* __restrict__ S, double* __restrict__ H0,
+                                              double* __restrict__ gam) {
+    __shared__ double s_alpha[XTB_MAX_US_LDS * XTB_MAXPRIM];
+    __shared__ double s_coeff[XTB_MAX_US_LDS * XTB_MAXPRIM];
+    stage_prims(par, s_alpha, s_coeff);
+    const int* tl = b.tile_sh + 3 * blockIdx.x;
+    const int m = tl[0];
+    const int i = tl[1] + threadIdx.x / 16;
+    const int j = tl[2] + threadIdx.x % 16;
+    const int nsh = b.mol_nsh[m];
+    if (i >= nsh || j >= nsh || j < i) return;
+    const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
+    const int ati = b.sh_atom[sh0 + i], atj = b.sh_atom[sh0 + j];
+    shell_pair_s_h0(b, par, pos, S, H0, m, i, j, cn[at0 + ati], cn[at0 + atj], s_alpha, s_coeff);
     // the same shell pair's ES2 entry (one launch fewer per batch)
     if (gam) gamma_pair(b, par, pos, gam, m, i, j, nsh);
 }
@@ -177,40 +207,99 @@ __global__ void __launch_bounds__(256) k_gamma(xtb_batch b, xtb_params par,
 // ---------------------------------------------------------------------------
 // repulsion energy (repulsion/base.py:269-334, rep.py:65-98):
 //   E = sum_{A<B, R<=25} Z_A Z_B exp(-sqrt(a_A a_B) R^kexp) / R
+// of molecule m by one workgroup, written (not added): thread t takes the
+// atom pairs t, t + blockDim, ... of the upper triangle in order and the
+// partial sums are added in a fixed order, so the energy has the same bits
+// on every run.
 // ---------------------------------------------------------------------------
-__global__ void k_zero_rep(xtb_batch b, double* out) {
-    const int m = blockIdx.x * blockDim.x + threadIdx.x;
-    if (m < b.nmol) out[(size_t)m * XTB_OUT_STRIDE + XTB_OUT_EREP] = 0.0;
-}
-
-__global__ void __launch_bounds__(256) k_repulsion(xtb_batch b, xtb_params par,
-                                                   const double* __restrict__ pos, double* out) {
-    const int* tl = b.tile_at + 3 * blockIdx.x;
-    const int m = tl[0];
-    const int A = tl[1] + threadIdx.x / 16;
-    const int B = tl[2] + threadIdx.x % 16;
-    const int nat = b.mol_nat[m];
+__device__ void repulsion_molecule(const xtb_batch& b, const xtb_params& par,
+                                   const double* __restrict__ pos, double* out, int m,
+                                   double* red) {
+    const int na = b.mol_nat[m], a0 = b.mol_atom0[m];
+    const long np = (long)na * (na - 1) / 2;
     double e = 0.0;
-    if (A < nat && B < nat && B > A) {
-        const int at0 = b.mol_atom0[m];
-        const int ga = at0 + A, gb = at0 + B;
+    for (long p = threadIdx.x; p < np; p += blockDim.x) {
+        // row A of the upper triangle holding pair index p (row A starts at
+        // A (2 na - A - 1) / 2)
+        int A = (int)((2.0 * na - 1.0 - sqrt((2.0 * na - 1.0) * (2.0 * na - 1.0) - 8.0 * (double)p)) * 0.5);
+        if (A < 0) A = 0;
+        while (A > 0 && (long)A * (2 * na - A - 1) / 2 > p) --A;
+        while ((long)(A + 1) * (2 * na - A - 2) / 2 <= p) ++A;
+        const int B = A + 1 + (int)(p - (long)A * (2 * na - A - 1) / 2);
+        const int ga = a0 + A, gb = a0 + B;
         const double dx = pos[3 * ga] - pos[3 * gb], dy = pos[3 * ga + 1] - pos[3 * gb + 1],
                      dz = pos[3 * ga + 2] - pos[3 * gb + 2];
         const double r = sqrt(dx * dx + dy * dy + dz * dz);
         if (r <= par.rep_cutoff) {
             const int sa = b.atom_sp[ga], sb = b.atom_sp[gb];
             const double alpha = sqrt(par.sp_arep[sa] * par.sp_arep[sb] + 2.2250738585072014e-308);
-            e = par.sp_zeff[sa] * par.sp_zeff[sb] * exp(-alpha * pow(r, par.kexp)) / r;
+            e += par.sp_zeff[sa] * par.sp_zeff[sb] * exp(-alpha * pow(r, par.kexp)) / r;
         }
     }
-    // one molecule per tile: reduce in the workgroup, one atomic per tile
-    __shared__ double red[4];
-    double v = e;
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = v;
+    for (int o = 32; o > 0; o >>= 1) e += __shfl_xor(e, o, 64);
+    if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = e;
     __syncthreads();
-    if (threadIdx.x == 0)
-        atomicAdd(out + (size_t)m * XTB_OUT_STRIDE + XTB_OUT_EREP, red[0] + red[1] + red[2] + red[3]);
+    if (threadIdx.x == 0) {
+        double tot = 0.0;
+        for (int w = 0; w < (int)blockDim.x / 64; ++w) tot += red[w];
+        out[(size_t)m * XTB_OUT_STRIDE + XTB_OUT_EREP] = tot;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_repulsion_mol(xtb_batch b, xtb_params par,
+                                                       const double* __restrict__ pos, double* out) {
+    __shared__ double red[4];
+    repulsion_molecule(b, par, pos, out, blockIdx.x, red);
+}
+
+// ---------------------------------------------------------------------------
+// The whole integral pass in one launch: workgroups [0, ntile_sh) are the
+// shell-pair tiles of k_s_h0 (S, H0 and gamma), each first forming the
+// coordination numbers of its tile's atoms in LDS (the same summation as
+// k_cn, so the same bits; the diagonal tile holding an atom's first shell
+// also stores it); workgroups [ntile_sh, ntile_sh + nmol) each write one
+// molecule's repulsion energy.  One launch instead of four (CN, S/H0/gamma,
+// zeroing, repulsion): under a busy serving loop every launch waits for
+// free compute units.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_integrals(xtb_batch b, xtb_params par,
+                                                   const double* __restrict__ pos,
+                                                   double* __restrict__ cn, double* __restrict__ S,
+                                                   double* __restrict__ H0, double* __restrict__ gam,
+                                                   double* __restrict__ out) {
+    __shared__ double s_alpha[XTB_MAX_US_LDS * XTB_MAXPRIM];
+    __shared__ double s_coeff[XTB_MAX_US_LDS * XTB_MAXPRIM];
+    __shared__ double s_cn[32];
+    if ((int)blockIdx.x >= b.ntile_sh) {
+        repulsion_molecule(b, par, pos, out, blockIdx.x - b.ntile_sh, s_cn);
+        return;
+    }
+    const int* tl = b.tile_sh + 3 * blockIdx.x;
+    const int m = tl[0], i0 = tl[1], j0 = tl[2];
+    const int nsh = b.mol_nsh[m];
+    const int sh0 = b.mol_sh0[m], at0 = b.mol_atom0[m];
+    // atoms of the tile's row shells [i0, i0+16) and column shells [j0, j0+16)
+    const int ra0 = b.sh_atom[sh0 + i0], ra1 = b.sh_atom[sh0 + min(i0 + 15, nsh - 1)];
+    const int ca0 = b.sh_atom[sh0 + j0], ca1 = b.sh_atom[sh0 + min(j0 + 15, nsh - 1)];
+    {
+        const int slot = threadIdx.x / CN_PARTS, part = threadIdx.x % CN_PARTS;
+        const int la = slot < 16 ? ra0 + slot : ca0 + (slot - 16);
+        const bool ok = slot < 16 ? la <= ra1 : la <= ca1;
+        const double v = cn_combine(ok ? cn_part(b, par, pos, at0 + la, part) : 0.0);
+        if (ok && part == 0) {
+            s_cn[slot] = v;
+            // the diagonal tile whose rows hold the atom's first shell stores it
+            if (slot < 16 && i0 == j0 && b.atom_sh0[at0 + la] >= i0) cn[at0 + la] = v;
+        }
+    }
+    __syncthreads();
+    stage_prims(par, s_alpha, s_coeff);
+    const int i = i0 + threadIdx.x / 16;
+    const int j = j0 + threadIdx.x % 16;
+    if (i >= nsh || j >= nsh || j < i) return;
+    const int ati = b.sh_atom[sh0 + i], atj = b.sh_atom[sh0 + j];
+    shell_pair_s_h0(b, par, pos, S, H0, m, i, j, s_cn[ati - ra0], s_cn[16 + atj - ca0], s_alpha, s_coeff);
+    gamma_pair(b, par, pos, gam, m, i, j, nsh);
 }
 
 // repulsion gradient (repulsion/base.py:337-406), one thread per atom A
@@ -276,8 +365,8 @@ static int32_t build_s_h0(const xtb_batch* batch, const xtb_params* par, const d
     if (batch->nmol <= 0 || batch->nat_total <= 0) return XTB_EINVAL;
     if (par->nus > XTB_MAX_US_LDS) return XTB_ETOOBIG;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_cn, dim3((batch->nat_total + 255) / 256), dim3(256), 0, s, *batch, *par,
-                       positions, cn);
+    hipLaunchKernelGGL(k_cn, dim3((batch->nat_total * CN_PARTS + 255) / 256), dim3(256), 0, s, *batch,
+                       *par, positions, cn);
     if (batch->ntile_sh > 0)
         hipLaunchKernelGGL(k_s_h0, dim3(batch->ntile_sh), dim3(256), 0, s, *batch, *par, positions,
                            cn, S, H0, gamma);
@@ -316,11 +405,22 @@ extern "C" int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
                                  const double* positions, double* out, void* stream) {
     if (!batch || !par || !positions || !out) return XTB_EINVAL;
     if (batch->nmol <= 0) return XTB_EINVAL;
+    hipLaunchKernelGGL(k_repulsion_mol, dim3(batch->nmol), dim3(256), 0, (hipStream_t)stream, *batch,
+                       *par, positions, out);
+    return check_launch();
+}
+
+extern "C" int32_t xtb_build_integrals(const xtb_batch* batch, const xtb_params* par,
+                                       const double* positions, double* S, double* H0, double* cn,
+                                       double* gamma, double* out, void* stream) {
+    if (!batch || !par || !positions || !S || !H0 || !cn || !gamma || !out) return XTB_EINVAL;
+    if (batch->nmol <= 0 || batch->nat_total <= 0 || batch->ntile_sh <= 0) return XTB_EINVAL;
+    if (par->nus > XTB_MAX_US_LDS) return XTB_ETOOBIG;
     hipStream_t s = (hipStream_t)stream;
-    hipLaunchKernelGGL(k_zero_rep, dim3((batch->nmol + 255) / 256), dim3(256), 0, s, *batch, out);
-    if (batch->ntile_at > 0)
-        hipLaunchKernelGGL(k_repulsion, dim3(batch->ntile_at), dim3(256), 0, s, *batch, *par,
-                           positions, out);
+    hipLaunchKernelGGL(k_integrals, dim3(batch->ntile_sh + batch->nmol), dim3(256), 0, s, *batch, *par,
+                       positions, cn, S, H0, gamma, out);
+    if (batch->has_padding)
+        hipLaunchKernelGGL(k_pad_fill, dim3(8, batch->nmol), dim3(256), 0, s, *batch, S, H0);
     return check_launch();
 }
 

@@ -379,7 +379,8 @@ class Pipeline:
 
         With ``timing`` HIP events bracket the SCF launch on its stream
         (``self.scf_events``) and the integral pass (CN, S, H0, gamma:
-        ``self.int_events``) for kernel-duration measurements.  ``wg_skip``
+        ``self.int_events``, which includes the repulsion energy) for
+        kernel-duration measurements.  ``wg_skip``
         (0..7) shifts the SCF launch's workgroups to later XCDs."""
         if positions.dtype != torch.float64 or not positions.is_contiguous():
             raise ValueError("positions must be contiguous float64")
@@ -397,13 +398,12 @@ class Pipeline:
         if timing:
             ei0, ei1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             ei0.record(st)
-        _lib.check(lib.xtb_build_s_h0_gamma(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
-                                            self.cn.data_ptr(), self.gamma.data_ptr(), s),
-                   "xtb_build_s_h0_gamma")
+        _lib.check(lib.xtb_build_integrals(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
+                                           self.cn.data_ptr(), self.gamma.data_ptr(),
+                                           self.out.data_ptr(), s), "xtb_build_integrals")
         if timing:
             ei1.record(st)
             self.int_events = (ei0, ei1)
-        _lib.check(lib.xtb_repulsion(b, p, pos, self.out.data_ptr(), s), "xtb_repulsion")
         # a warm start needs the state a previous keep_state launch on these
         # buffers left (in stream order, so the launches need no other sync)
         if self._kept and opts.warm_extrapolate:

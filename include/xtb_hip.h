@@ -291,6 +291,16 @@ int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par,
 int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
                       const double* positions, double* out, void* stream);
 
+/* The whole integral pass in one launch: CN, S, H0 (xtb_build_s_h0), gamma
+ * (xtb_build_gamma) and the repulsion energy (xtb_repulsion), with the same
+ * results bit for bit.  Replaces the same upstream call sites as those three:
+ * GFN1Hamiltonian.build (_src/xtb/base.py:251-360) with cn_d3, ES2.get_cache
+ * (coulomb/secondorder.py:216-276) and Repulsion.get_energy
+ * (classicals/repulsion/base.py:269-334). */
+int32_t xtb_build_integrals(const xtb_batch* batch, const xtb_params* par,
+                            const double* positions, double* S, double* H0, double* cn,
+                            double* gamma, double* out, void* stream);
+
 /* Gradient of the repulsion energy alone, per atom [3*nat_total] (written,
  * not added); replaces the autograd of Repulsion.get_energy
  * (src/dxtb/_src/components/classicals/repulsion/base.py:337-406).  With

@@ -335,8 +335,9 @@ def test_xcd_shift_changes_placement_not_results(fx):
 
 
 def test_fused_integrals_equal_the_separate_entry_points(fx):
-    """xtb_build_s_h0_gamma (what the pipeline calls) writes exactly what
-    xtb_build_s_h0 and xtb_build_gamma write."""
+    """xtb_build_integrals (what the pipeline calls: CN, S, H0, gamma and the
+    repulsion in one launch) writes exactly what xtb_build_s_h0,
+    xtb_build_gamma, xtb_build_s_h0_gamma and xtb_repulsion write."""
     import ctypes as ct
 
     from dxtb_amd import _lib
@@ -352,15 +353,26 @@ def test_fused_integrals_equal_the_separate_entry_points(fx):
                                   cn.data_ptr(), s), "xtb_build_s_h0")
     _lib.check(lib.xtb_build_gamma(ct.byref(p.batch), ct.byref(p.dt.s), pos, gam.data_ptr(), s),
                "xtb_build_gamma")
+    out = torch.full_like(p.out, float("nan"))
+    _lib.check(lib.xtb_repulsion(ct.byref(p.batch), ct.byref(p.dt.s), pos, out.data_ptr(), s),
+               "xtb_repulsion")
+    S2, H2, g2, c2 = (torch.full_like(t, float("nan")) for t in (p.S, p.H0, p.gamma, p.cn))
+    _lib.check(lib.xtb_build_s_h0_gamma(ct.byref(p.batch), ct.byref(p.dt.s), pos, S2.data_ptr(),
+                                        H2.data_ptr(), c2.data_ptr(), g2.data_ptr(), s),
+               "xtb_build_s_h0_gamma")
     torch.cuda.synchronize()
-    assert torch.equal(cn, p.cn)
+    assert torch.equal(cn, p.cn) and torch.equal(c2, p.cn)
+    assert torch.equal(out.view(-1, 32)[:, 9], p.records()[:, 9])
     for b in range(len(mols)):
         # each molecule's blocks (the packed buffers keep alignment gaps
         # between molecules that no kernel writes)
         assert torch.equal(p.matrix(S, b), p.matrix(p.S, b))
         assert torch.equal(p.matrix(H0, b), p.matrix(p.H0, b))
+        assert torch.equal(p.matrix(S2, b), p.matrix(p.S, b))
+        assert torch.equal(p.matrix(H2, b), p.matrix(p.H0, b))
         g0, m = int(p.L.gam0[b]), int(p.L.nsh[b])
         assert torch.equal(gam[g0:g0 + m * m], p.gamma[g0:g0 + m * m])
+        assert torch.equal(g2[g0:g0 + m * m], p.gamma[g0:g0 + m * m])
 
 
 def test_extrapolated_warm_start_reaches_the_same_fixed_point(fx):
