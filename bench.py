@@ -180,6 +180,41 @@ def md_warm_start(device, opts, steps: int = 8) -> dict:
             "speedup_extrapolated": round(out["cold"][0] / out["warm_extrapolated"][0], 3)}
 
 
+def large_system(device, opts, repeats: int = 3) -> dict:
+    """Config 3: one 600-water cluster (1800 atoms, 4800 orbitals), energy +
+    analytic forces on the whole GPU (xtb_scf_large).  One warm-up run, then
+    `repeats` timed runs (host clock around the whole call, synchronised);
+    the median, with the SCF's own stage times from its result record."""
+    import torch
+
+    from dxtb_amd import _lib
+    from dxtb_amd import workloads as W
+    from dxtb_amd.large import LargeSystem
+
+    z, p = W.config3()
+    ls = LargeSystem(z, device, opts=opts)
+    pos = torch.as_tensor(p, dtype=torch.float64, device=device)
+    r = ls.run(pos, forces=True)
+    times = []
+    for _ in range(repeats):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = ls.run(pos, forces=True)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    rec = r["records"]
+    ms = sorted(times)[len(times) // 2] * 1e3
+    return {"system": "600 H2O (1800 atoms, 4800 orbitals)", "ms": round(ms, 1),
+            "repeats_ms": [round(t * 1e3, 1) for t in times],
+            "energy": r["energy"], "iterations": r["iterations"], "status": r["status"],
+            "scf_ms": round(float(rec[_lib.OUT_T_TOTAL]) * 1e-3, 1),
+            "orthogonaliser_ms": round(float(rec[_lib.OUT_T_ORTHO]) * 1e-3, 1),
+            "eigen_ms": round(float(rec[_lib.OUT_T_EIGEN]) * 1e-3, 1),
+            "perturbative_ms": round(float(rec[_lib.OUT_T_PD]) * 1e-3, 1),
+            "perturbative_steps": int(rec[_lib.OUT_PD_STEPS]),
+            "jacobi_sweeps": int(rec[_lib.OUT_SWEEPS])}
+
+
 def device_peaks(device) -> dict:
     """Achievable FP64 GEMM rate (torch.matmul, n=8192) and device copy
     bandwidth (1 GiB), for context next to the published peaks."""
@@ -362,6 +397,9 @@ def main() -> int:
                     help="serve one chunk only (for rocprofv3 runs), print nothing else")
     ap.add_argument("--profile-serving", action="store_true",
                     help="only serve 8 chunks through the continuous-batching loop (kernel traces)")
+    ap.add_argument("--no-large", action="store_true", help="skip the 600-water single system")
+    ap.add_argument("--profile-large", action="store_true",
+                    help="only the 600-water single system, energy + forces once (kernel traces)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (HBM counter calibration)")
     a = ap.parse_args()
@@ -399,6 +437,9 @@ def main() -> int:
                           "bytes_written": y.numel() * 8}))
         return 0
     opts = ScfOptions(x_atol=1e-10, x_atol_max=1e-10, maxiter=300)
+    if a.profile_large:
+        print(json.dumps({"profile_large": large_system(dev, opts, repeats=1)}))
+        return 0
     total = max((a.warmup + a.steps) * a.chunk, a.latency_requests, ROOFLINE_START + ROOFLINE_COUNT)
     stream = W.config4(seed=4, count=total)
     log(f"stream: {total} requests generated")
@@ -537,6 +578,8 @@ def main() -> int:
     log(f"forces cost: {fc}")
     md = md_warm_start(dev, opts)
     log(f"md warm start: {md}")
+    big = None if a.no_large else large_system(dev, opts)
+    log(f"600-water cluster: {big}")
     peaks = device_peaks(dev)
     log(f"measured peaks: {peaks}")
 
@@ -571,6 +614,7 @@ def main() -> int:
             "latency_rate_per_s": head["rate_per_s"], "p50_ms": head["p50_ms"],
             "p99_ms": head["p99_ms"], "latency_curve": curve, "nonconverged": bad,
             "forces_over_energy": fc, "md_warm_start": md,
+            "single_large_system": big,
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),

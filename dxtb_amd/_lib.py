@@ -30,7 +30,8 @@ MOL_OVERLAP = 16
 OUT_STRIDE = 32
 OUT_ESCF, OUT_EH0, OUT_EES2, OUT_EES3, OUT_FENERGY = 0, 1, 2, 3, 4
 OUT_ITER, OUT_STATUS, OUT_EF_A, OUT_EF_B, OUT_EREP, OUT_SWEEPS = 5, 6, 7, 8, 9, 10
-OUT_T_TOTAL = 14  # SCF kernel device time of the molecule, microseconds
+OUT_T_ORTHO, OUT_T_EIGEN, OUT_T_TOTAL = 11, 13, 14  # stage and SCF times of the molecule, microseconds
+OUT_PD_STEPS, OUT_SWEEPS_FIRST, OUT_SWEEPS_LAST, OUT_T_PD = 17, 18, 19, 21
 OUT_T_START = 20  # device wall clock at the molecule's start, microseconds
 OUT_XCD = 22  # XCD the molecule's workgroup ran on
 OUT_T_FOCK, OUT_N_FOCK, OUT_T_MULL, OUT_N_MULL = 24, 25, 26, 27  # stage timers (us) and counts
@@ -133,6 +134,10 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_param_grad_basis.restype = i32
     lib.xtb_param_grad_basis.argtypes = [bp, pp, P, P, P, P, ct.c_uint32, ct.c_double, P, P,
                                          ct.POINTER(XtbScfState), P, i32, i32, P]
+    lib.xtb_scf_large_workspace.restype = i32
+    lib.xtb_scf_large_workspace.argtypes = [bp, ct.POINTER(i64)]
+    lib.xtb_scf_large.restype = i32
+    lib.xtb_scf_large.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfOpts), ct.POINTER(XtbScfState), P, i64, P]
     if lib.xtb_abi_version() != 1:
         raise NativeLibraryError("libxtbhip.so ABI version mismatch")
     sizes = (i64 * 4)()
@@ -148,7 +153,7 @@ EXPORTED = (
     "xtb_abi_version", "xtb_abi_struct_sizes", "xtb_workspace_doubles", "xtb_build_s_h0", "xtb_build_gamma",
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
     "xtb_param_grad_basis", "xtb_build_s_h0_gamma", "xtb_repulsion_gradient",
-    "xtb_forces_noalloc", "xtb_build_integrals",
+    "xtb_forces_noalloc", "xtb_build_integrals", "xtb_scf_large_workspace", "xtb_scf_large",
 )
 
 

@@ -44,8 +44,8 @@ from .exceptions import (
 from .param import SpeciesTables
 
 _DISP_WARNED = False
-# single molecules with more orbitals than this run on the whole-GPU path
-# (dxtb_amd.large); batches use the per-molecule kernel (limit 1024)
+# a single molecule with more orbitals than this runs its SCF on the whole
+# GPU (xtb_scf_large); batches use the per-molecule kernel (limit 1024)
 LARGE_N = 600
 
 
@@ -183,7 +183,9 @@ class GFN1Calculator:
                 self._dtables = DeviceTables(self.tables, self.device)
             layout = BatchLayout(self._mols, self.tables, padded=self.batched and self.layout_padded,
                                  chrg=c, spin=sp)
-            self._pipe = Pipeline(layout, self._dtables, self.device)
+            # one molecule above LARGE_N orbitals: its SCF on the whole GPU
+            large = layout.B == 1 and not layout.padded and int(layout.n[0]) > LARGE_N
+            self._pipe = Pipeline(layout, self._dtables, self.device, large=large)
             self._pipe_key = key
         return self._pipe
 
@@ -215,11 +217,6 @@ class GFN1Calculator:
         hit = self._cached(positions, ckey, forces, orbitals)
         if hit is not None:
             return hit
-        if not self.batched:
-            from .serving import orbital_count
-
-            if orbital_count(self._mols[0]) > LARGE_N:
-                return self._run_large(positions, chrg, spin, forces, ckey)
         pipe = self._pipeline(chrg, spin)
         pos = positions.detach()
         flat = pos[self._mask].contiguous() if self.batched else pos[self._mask].contiguous()
@@ -261,47 +258,6 @@ class GFN1Calculator:
         is zero for H/C/N/O)."""
         ex = self.exclude
         return not ({"scf", "all"} & ex), not ({"rep", "all"} & ex)
-
-    def _run_large(self, positions, chrg, spin, forces, ckey) -> dict:
-        from .large import LargeSystem
-
-        if spin is not None:
-            raise NotImplementedError("spin for the large-system path")
-        c = float(self._charges_array(chrg)[0])
-        if self._pipe_key != ("large", c):
-            self._pipe = LargeSystem(self._mols[0], self.device, opts=self.scf_opts, chrg=c)
-            self._pipe_key = ("large", c)
-        pos = positions.detach()[self._mask].contiguous()
-        r = self._pipe.run(pos, forces=forces)
-        if not r["converged"]:
-            msg = f"SCF does not converge after {self.scf_opts.maxiter} cycles."
-            if self.force_convergence:
-                raise SCFConvergenceError(msg)
-            warnings.warn(msg, SCFConvergenceWarning)
-        use_scf, use_rep = self._components()
-        e = torch.tensor((r["energy"] - r["repulsion"] if use_scf else 0.0)
-                         + (r["repulsion"] if use_rep else 0.0), dtype=torch.float64, device=self.device)
-        rec = torch.zeros((1, _lib.OUT_STRIDE), dtype=torch.float64, device=self.device)
-        rec[0, _lib.OUT_ESCF] = r["scf"]
-        rec[0, _lib.OUT_EREP] = r["repulsion"]
-        rec[0, _lib.OUT_ITER] = r["iterations"]
-        res = {"energy": e, "records": rec, "pipeline": self._pipe.pipe, "large": r}
-        if forces:
-            pipe = self._pipe.pipe
-            if use_scf and use_rep:
-                gm = r["gradient"]
-            elif use_scf:
-                gm = r["gradient"] - pipe.repulsion_gradient()
-            elif use_rep:
-                gm = pipe.repulsion_gradient()
-            else:
-                gm = torch.zeros_like(r["gradient"])
-            g = torch.zeros(positions.shape, dtype=torch.float64, device=self.device)
-            g[self._mask] = gm
-            res["gradient_full"] = g
-        # orbital energies are never formed here (get_mo_energies raises)
-        self._store(positions, ckey, forces, True, res)
-        return res
 
     def _raise_on_status(self, status: np.ndarray) -> None:
         if np.any(status & _lib.MOL_OVERLAP):
@@ -371,8 +327,6 @@ class GFN1Calculator:
     def get_charges(self, positions: torch.Tensor, chrg=0, spin=None, **kwargs) -> torch.Tensor:
         """Orbital-resolved Mulliken charges, shape (..., nao) (types/abc.py:429-486)."""
         res = self._run(positions, chrg, spin, forces=False)
-        if "large" in res:
-            return res["large"]["charges_orbital"].clone()
         pipe = res["pipeline"]
         L = pipe.L
         nmax = int(L.nao.max())
@@ -396,8 +350,6 @@ class GFN1Calculator:
         from .param import elements_by_z
 
         res = self._run(positions, chrg, spin, forces=True)
-        if "large" in res:
-            raise PropertyNotImplementedError("parameter gradients on the large-system path")
         g = res["pipeline"].param_grad()
         tb = self.tables
         d_zeta = res["pipeline"].param_grad_basis(tb)
@@ -452,9 +404,6 @@ class GFN1Calculator:
 
     def get_mo_energies(self, positions, chrg=0, spin=None):
         res = self._run(positions, chrg, spin, forces=False, orbitals=True)
-        if "large" in res:
-            raise PropertyNotImplementedError(
-                "orbital energies: the large-system path builds the density by purification")
         pipe = res["pipeline"]
         L = pipe.L
         out = [pipe.emo[int(L.vec0[b]): int(L.vec0[b]) + int(L.n[b])].clone() for b in range(L.B)]

@@ -1,0 +1,256 @@
+// Per-molecule SCF pieces shared by the one-workgroup-per-molecule kernel
+// (scf.hip) and the whole-GPU path for one large molecule (large.hip): the
+// molecule view, the LDS view, Fermi smearing and the Anderson mixer.  Every
+// function here is called by all threads of one workgroup.
+#pragma once
+
+#include "common.h"
+#include "wg.h"
+
+namespace xtb {
+
+struct Mol {
+    int m, n, nao, nsh, nat, vec0, sh0, at0;
+    size_t mat0, gam0;
+    const glbf *S, *H0, *gam;
+    glbf *F, *Ct, *T, *P;
+    glbf *pot, *vnew, *q, *eorb, *emo, *occ, *qsh_out;
+    double nel_a, nel_b;
+    int nvec_total;
+};
+
+struct Shared {
+    ldsf *red, *cs, *skey, *ev, *ftot, *qsh, *vsh, *gq, *qat, *big;
+    ldsi* sidx;
+    int npow, big_cap;
+    double* tacc;  // per-thread stage timers (meaningful on thread 0)
+#ifdef XTB_PD_TIMERS
+    double* ptacc;  // phases of the perturbative step (instrumented builds only)
+#endif
+};
+
+#ifdef XTB_PD_TIMERS
+#define PD_MARK(sh, i, t) do { const double t_ = now_us(); (sh).ptacc[i] += t_ - (t); (t) = t_; } while (0)
+#else
+#define PD_MARK(sh, i, t) do { } while (0)
+#endif
+
+// device wall clock (constant 100 MHz) in microseconds
+__device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
+
+// electronic temperatures below this (0.1 K in Hartree) fill the orbitals
+// by the aufbau principle instead of Fermi smearing
+// (src/dxtb/_src/scf/base.py:888-889)
+constexpr double KT_AUFBAU = 3e-7;
+
+// ---------------------------------------------------------------------------
+// Fermi smearing of one spin channel (src/dxtb/_src/wavefunction/
+// filling.py:201-366): mid-gap start, Newton on the rounded electron count
+// (scf/base.py:879), exponent cutoff 50, threshold on |N - sum f|.  Below
+// KT_AUFBAU the aufbau occupations of the unrounded count instead
+// (filling.py:116-198; base.py:888-889 keeps the initial filling of
+// iterator.py:147-189): orbital j (from 0) is full while j + 1 < nel, the
+// next one takes nel - ceil(nel) + 1.  Sorted eigenvalues in sh.skey.
+// Writes the sorted occupations to occ_out and adds them to sh.ftot.
+// ---------------------------------------------------------------------------
+template <class SH>
+__device__ __noinline__ bool fermi_channel(const Mol& M, const SH& sh, double nel, double kt, double thr,
+                              int maxiter, bool padded, glbf* occ_out, double* efermi) {
+    const int n = M.n;
+    const double eps = 2.220446049250313e-16;
+    if (fabs(nel) < eps) {
+        for (int j = threadIdx.x; j < n; j += NT) occ_out[j] = 0.0;
+        *efermi = 0.0;
+        __syncthreads();
+        return true;
+    }
+    if (kt < KT_AUFBAU) {
+        const double frac = nel - ceil(nel) + 1.0;
+        for (int j = threadIdx.x; j < n; j += NT) {
+            const double idx = (double)(j + 1);
+            const double f = idx < nel ? 1.0 : (idx < nel + 1.0 ? frac : 0.0);
+            occ_out[j] = f;
+            sh.ftot[sh.sidx[j]] += f;
+        }
+        *efermi = 0.0;
+        __syncthreads();
+        return true;
+    }
+    nel = rint(nel);
+    int homo = (int)ceil(nel - 5e-15) - 1;
+    if (homo < 0) homo = 0;
+    if (homo > n - 1) homo = n - 1;
+    const bool lumo_missing = (n - 1 <= homo);
+    const int lumo = lumo_missing ? homo : homo + 1;
+    double ef = 0.5 * (sh.skey[homo] + sh.skey[lumo]);
+    if (padded && (homo >= M.nao || lumo >= M.nao)) ef *= 2.0;  // filling.py:254-264
+    const double target = (double)(homo + 1);
+    for (int it = 0; it < maxiter; ++it) {
+        double acc[2] = {0.0, 0.0};
+        for (int j = threadIdx.x; j < n; j += NT) {
+            const double x = (sh.skey[j] - ef) / kt;
+            if (x < 50.0) {
+                const double et = exp(x);
+                acc[0] += 1.0 / (et + 1.0);
+                acc[1] += et / (kt * (et + 1.0) * (et + 1.0));
+            } else {
+                acc[1] += eps;
+            }
+        }
+        wg_sum_n<2>(acc, sh.red);
+        const double diff = target - acc[0];
+        if (fabs(diff) <= thr) {
+            for (int j = threadIdx.x; j < n; j += NT) {
+                const double x = (sh.skey[j] - ef) / kt;
+                const double f = (x < 50.0) ? 1.0 / (exp(x) + 1.0) : 0.0;
+                occ_out[j] = f;
+                sh.ftot[sh.sidx[j]] += f;
+            }
+            *efermi = ef;
+            __syncthreads();
+            return true;
+        }
+        ef += diff / acc[1];
+    }
+    for (int j = threadIdx.x; j < n; j += NT) occ_out[j] = 0.0;
+    *efermi = ef;
+    __syncthreads();
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Anderson mixing with soft start (scf/mixer/anderson.py:163-317), history in
+// a ring buffer of G = generations+1 slots.  Returns the convergence flag of
+// the step (mixer/base.py:229-256 on the step's residual).
+// ---------------------------------------------------------------------------
+// largest history (generations) the mixer takes; its Gram system's 27 sums
+// fit one pass of the reduction scratch, and 8 generations (44 sums) grow
+// the kernel's stack by ~0.9 KB per lane for ~1 % fewer iterations
+constexpr int MIX_MAXG = 6;
+
+struct Mixer {
+    int step, head, G;
+    bool soft;  // upstream's soft start; a warm start, already near the fixed point, skips it
+};
+
+template <class SH>
+__device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const SH& sh, const xtb_scf_opts& o,
+                         glbf* hx, glbf* hf, const glbf* xnew, const glbf* xold,
+                         glbf* xout, double* resid_inf) {
+    const int n = M.n, G = mx.G;
+    auto X = [&](int h) { return hx + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
+    auto Fh = [&](int h) { return hf + (size_t)((mx.head + h) % G) * M.nvec_total + M.vec0; };
+    if (mx.step == 0) {
+        glbf* x0 = X(0);
+        for (int i = threadIdx.x; i < n; i += NT) x0[i] = xold[i];
+        __syncthreads();
+    }
+    mx.step += 1;
+    const int gen = G - 1;
+    glbf* x0 = X(0);
+    glbf* f0 = Fh(0);
+    double l2 = 0.0, linf = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double f = xnew[i] - x0[i];
+        f0[i] = f;
+        l2 = fma(f, f, l2);
+        linf = fmax(linf, fabs(f));
+    }
+    l2 = wg_sum(l2, sh.red);
+    linf = wg_max(linf, sh.red);
+    const bool conv = (sqrt(l2) < o.x_atol) && (linf < o.x_atol_max);
+    *resid_inf = linf;
+    double theta[MIX_MAXG];
+#pragma unroll
+    for (int h = 0; h < MIX_MAXG; ++h) theta[h] = 0.0;
+    int nn = 0;
+    if (mx.step > gen || !mx.soft) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
+    if (nn > 0) {
+        // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn
+        constexpr int NA = MIX_MAXG * (MIX_MAXG + 1) / 2, NACC = NA + MIX_MAXG;
+        double acc[NACC];
+#pragma unroll
+        for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
+        for (int i = threadIdx.x; i < n; i += NT) {
+            double d[MIX_MAXG];
+            const double fi = f0[i];
+#pragma unroll
+            for (int h = 0; h < MIX_MAXG; ++h) d[h] = (h < nn) ? fi - Fh(h + 1)[i] : 0.0;
+            int k = 0;
+#pragma unroll
+            for (int h = 0; h < MIX_MAXG; ++h)
+#pragma unroll
+                for (int j = h; j < MIX_MAXG; ++j) acc[k++] += d[h] * d[j];
+#pragma unroll
+            for (int h = 0; h < MIX_MAXG; ++h) acc[NA + h] += d[h] * fi;
+        }
+        static_assert(NACC * NWAVE <= 256, "one pass of the reduction scratch");
+        wg_sum_n<NACC>(acc, sh.red);
+        // solve (a * diag(1 + off^2)) theta = b by Gaussian elimination with
+        // partial pivoting; one thread, system held in LDS
+        constexpr int W = MIX_MAXG + 1;
+        ldsf* a = sh.red;                  // [MIX_MAXG][W]
+        ldsf* th = sh.red + MIX_MAXG * W;  // [MIX_MAXG]
+        if (threadIdx.x == 0) {
+            int k = 0;
+#pragma unroll
+            for (int h = 0; h < MIX_MAXG; ++h)
+#pragma unroll
+                for (int j = h; j < MIX_MAXG; ++j) { a[h * W + j] = acc[k]; a[j * W + h] = acc[k]; ++k; }
+#pragma unroll
+            for (int h = 0; h < MIX_MAXG; ++h) a[h * W + MIX_MAXG] = acc[NA + h];
+            for (int h = 0; h < nn; ++h) a[h * W + h] *= 1.0 + o.diag_offset * o.diag_offset;
+            for (int c = 0; c < nn; ++c) {
+                int piv = c;
+                for (int r = c + 1; r < nn; ++r)
+                    if (fabs(a[r * W + c]) > fabs(a[piv * W + c])) piv = r;
+                if (piv != c)
+                    for (int x = 0; x < W; ++x) {
+                        const double t = a[c * W + x];
+                        a[c * W + x] = a[piv * W + x];
+                        a[piv * W + x] = t;
+                    }
+                for (int r = c + 1; r < nn; ++r) {
+                    const double fct = a[r * W + c] / a[c * W + c];
+                    for (int x = c; x < W; ++x) a[r * W + x] -= fct * a[c * W + x];
+                }
+            }
+            for (int r = nn - 1; r >= 0; --r) {
+                double sacc = a[r * W + MIX_MAXG];
+                for (int x = r + 1; x < nn; ++x) sacc -= a[r * W + x] * th[x];
+                th[r] = sacc / a[r * W + r];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < MIX_MAXG; ++h) theta[h] = (h < nn) ? th[h] : 0.0;
+        __syncthreads();
+    }
+    // new slot for the mixed vector = oldest slot
+    const int newhead = (mx.head - 1 + G) % G;
+    glbf* xn = hx + (size_t)newhead * M.nvec_total + M.vec0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        double xm;
+        if (nn > 0) {
+            double xb = x0[i], fb = f0[i];
+#pragma unroll
+            for (int h = 0; h < MIX_MAXG; ++h) {
+                if (h < nn) {
+                    const double xh = X(h + 1)[i], fh = Fh(h + 1)[i];
+                    xb += theta[h] * (xh - x0[i]);
+                    fb -= theta[h] * (f0[i] - fh);
+                }
+            }
+            xm = xb + o.damp * fb;
+        } else {
+            xm = x0[i] + (mx.soft ? o.damp_init : o.damp) * f0[i];
+        }
+        xout[i] = xm;
+        xn[i] = xm;  // slot newhead may alias X(G-1): element-wise read-before-write
+    }
+    __syncthreads();
+    mx.head = newhead;
+    return conv;
+}
+
+}  // namespace xtb

@@ -122,38 +122,26 @@ __device__ inline v4d v4zero() {
 #define XTB_GEMM_SPLIT(TA, TB) (true)
 #endif
 
-// With `upper` (M == N, a product known to be symmetric) only the tiles on
-// and above the diagonal are formed; gemm_sym mirrors the rest.
-// With EPI, C = alpha op(A) op(B) + s1 E1 + s2 E2 (E1, E2 with C's leading
+// One TM x TN output tile at (m0, n0).  With EPI, C = alpha op(A) op(B) + s1 E1 + s2 E2 (E1, E2 with C's leading
 // dimension; beta unused): an elementwise update fused into the epilogue.
 template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false>
-__device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
-                                          int lda, const glbf* __restrict__ B, int ldb, double beta,
-                                          glbf* __restrict__ C, int ldc, ldsf* lds, int upper,
-                                          const glbf* __restrict__ E1 = nullptr,
-                                          const glbf* __restrict__ E2 = nullptr, double s1 = 0.0,
-                                          double s2 = 0.0) {
+__device__ __forceinline__ void gemm_tile(int m0, int n0, int M, int N, int K, double alpha,
+                                          const glbf* __restrict__ A, int lda,
+                                          const glbf* __restrict__ B, int ldb, double beta,
+                                          glbf* __restrict__ C, int ldc, ldsf* lds,
+                                          const glbf* __restrict__ E1, const glbf* __restrict__ E2,
+                                          double s1, double s2) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
     constexpr int TM = 32 * FM, TN = 64 * FN;
     constexpr int LDA_S = TM + 1, LDB_S = TN + 1;
     constexpr int EA = TM * KP / NT, EB = TN * KP / NT;  // staged elements per thread
     static_assert(EA * NT == TM * KP && EB * NT == TN * KP, "panel split over the threads");
     constexpr int STAGE = KP * (LDA_S + LDB_S);
-    // arguments of an out-of-line call arrive in vector registers; they are
-    // uniform, and saying so turns the tile and fragment tests into scalar
-    // branches instead of exec-masked code
-    M = __builtin_amdgcn_readfirstlane(M);
-    N = __builtin_amdgcn_readfirstlane(N);
-    K = __builtin_amdgcn_readfirstlane(K);
-    lda = __builtin_amdgcn_readfirstlane(lda);
-    ldb = __builtin_amdgcn_readfirstlane(ldb);
-    ldc = __builtin_amdgcn_readfirstlane(ldc);
     const int t = threadIdx.x;
     // wave index made scalar, so that per-wave decisions are scalar branches
     const int w = __builtin_amdgcn_readfirstlane(t / WAVE);
     const int lane = t % WAVE, li = lane % 16, lg = lane / 16;
     const int wm = w / 4, wn = w % 4;
-    const int tm = (M + TM - 1) / TM, tn = (N + TN - 1) / TN;
     // element e of this thread: (row along the contiguous direction)
     auto a_mk = [&](int e, int& m, int& k) {
         const int idx = t + e * NT;
@@ -165,137 +153,162 @@ __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, con
         if (TB) { k = idx % KP; n = idx / KP; }       // B[n][k]: contiguous k
         else { n = idx % TN; k = idx / TN; }          // B[k][n]: contiguous n
     };
-    upper = __builtin_amdgcn_readfirstlane(upper);
-    for (int tile = 0; tile < tm * tn; ++tile) {
-        if (upper && tile / tn > tile % tn) continue;
-        const int m0 = (tile / tn) * TM, n0 = (tile % tn) * TN;
-        // 16x16 fragments of this wave that hold any output (the last tile
-        // row/column is partial): the others skip their MFMAs
-        const int rrem = M - (m0 + wm * (TM / 2)), crem = N - (n0 + wn * (TN / 4));
-        const int nfm = rrem <= 0 ? 0 : (rrem >= FM * 16 ? FM : (rrem + 15) / 16);
-        const int nfn = crem <= 0 ? 0 : (crem >= FN * 16 ? FN : (crem + 15) / 16);
-        const bool full = nfm == FM && nfn == FN;
-        v4d acc[FM][FN];
+    // 16x16 fragments of this wave that hold any output (the last tile
+    // row/column is partial): the others skip their MFMAs
+    const int rrem = M - (m0 + wm * (TM / 2)), crem = N - (n0 + wn * (TN / 4));
+    const int nfm = rrem <= 0 ? 0 : (rrem >= FM * 16 ? FM : (rrem + 15) / 16);
+    const int nfn = crem <= 0 ? 0 : (crem >= FN * 16 ? FN : (crem + 15) / 16);
+    const bool full = nfm == FM && nfn == FN;
+    v4d acc[FM][FN];
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < FN; ++j) acc[i][j] = v4zero();
-        double pa[EA], pb[EB];
-        auto fetch = [&](int k0) {
+        for (int j = 0; j < FN; ++j) acc[i][j] = v4zero();
+    double pa[EA], pb[EB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-            for (int e = 0; e < EA; ++e) {
-                int m, k;
-                a_mk(e, m, k);
-                const int gm = m0 + m, gk = k0 + k;
-                pa[e] = (gm < M && gk < K) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
-            }
-#pragma unroll
-            for (int e = 0; e < EB; ++e) {
-                int n, k;
-                b_nk(e, n, k);
-                const int gn = n0 + n, gk = k0 + k;
-                pb[e] = (gn < N && gk < K) ? (TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn]) : 0.0;
-            }
-        };
-        auto store = [&](ldsf* st) {
-            ldsf* As = st;
-            ldsf* Bs = st + KP * LDA_S;
-#pragma unroll
-            for (int e = 0; e < EA; ++e) {
-                int m, k;
-                a_mk(e, m, k);
-                As[k * LDA_S + m] = pa[e];
-            }
-#pragma unroll
-            for (int e = 0; e < EB; ++e) {
-                int n, k;
-                b_nk(e, n, k);
-                Bs[k * LDB_S + n] = pb[e];
-            }
-        };
-        // MFMAs of one panel when every fragment of the wave holds output
-        auto panel_full = [&](const ldsf* As, const ldsf* Bs) {
-#pragma unroll
-            for (int kk = 0; kk < KP / 4; ++kk) {
-                double a[FM], b[FN];
-#pragma unroll
-                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDA_S + wm * (TM / 2) + i * 16 + li];
-#pragma unroll
-                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDB_S + wn * (TN / 4) + j * 16 + li];
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
-                __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // DS reads
-                __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);  // MFMAs
-            }
-        };
-        // partial tiles: fragments without output skip their MFMAs
-        auto panel_part = [&](const ldsf* As, const ldsf* Bs) {
-#pragma unroll
-            for (int kk = 0; kk < KP / 4; ++kk) {
-                double a[FM], b[FN];
-#pragma unroll
-                for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDA_S + wm * (TM / 2) + i * 16 + li];
-#pragma unroll
-                for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDB_S + wn * (TN / 4) + j * 16 + li];
-#pragma unroll
-                for (int i = 0; i < FM; ++i)
-#pragma unroll
-                    for (int j = 0; j < FN; ++j)
-                        if (i < nfm && j < nfn) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
-            }
-        };
-        const int np = (K + KP - 1) / KP;
-        if (np > 0) {
-            fetch(0);
-            store(lds);
-            __syncthreads();
+        for (int e = 0; e < EA; ++e) {
+            int m, k;
+            a_mk(e, m, k);
+            const int gm = m0 + m, gk = k0 + k;
+            pa[e] = (gm < M && gk < K) ? (TA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]) : 0.0;
         }
-        if (XTB_GEMM_SPLIT(TA, TB)) {
-            // one panel loop per fragment pattern, so the accumulators are
-            // not merged between the two code paths after every panel
-            if (full) {
-                for (int p = 0; p < np; ++p) {
-                    ldsf* cur = lds + (p & 1) * STAGE;
-                    if (p + 1 < np) fetch((p + 1) * KP);
-                    panel_full(cur, cur + KP * LDA_S);
-                    if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
-                    __syncthreads();
-                }
-            } else {
-                for (int p = 0; p < np; ++p) {
-                    ldsf* cur = lds + (p & 1) * STAGE;
-                    if (p + 1 < np) fetch((p + 1) * KP);
-                    panel_part(cur, cur + KP * LDA_S);
-                    if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
-                    __syncthreads();
-                }
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+            int n, k;
+            b_nk(e, n, k);
+            const int gn = n0 + n, gk = k0 + k;
+            pb[e] = (gn < N && gk < K) ? (TB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn]) : 0.0;
+        }
+    };
+    auto store = [&](ldsf* st) {
+        ldsf* As = st;
+        ldsf* Bs = st + KP * LDA_S;
+#pragma unroll
+        for (int e = 0; e < EA; ++e) {
+            int m, k;
+            a_mk(e, m, k);
+            As[k * LDA_S + m] = pa[e];
+        }
+#pragma unroll
+        for (int e = 0; e < EB; ++e) {
+            int n, k;
+            b_nk(e, n, k);
+            Bs[k * LDB_S + n] = pb[e];
+        }
+    };
+    // MFMAs of one panel when every fragment of the wave holds output
+    auto panel_full = [&](const ldsf* As, const ldsf* Bs) {
+#pragma unroll
+        for (int kk = 0; kk < KP / 4; ++kk) {
+            double a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDA_S + wm * (TM / 2) + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDB_S + wn * (TN / 4) + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+            __builtin_amdgcn_sched_group_barrier(0x100, FM + FN, 0);  // DS reads
+            __builtin_amdgcn_sched_group_barrier(0x008, FM * FN, 0);  // MFMAs
+        }
+    };
+    // partial tiles: fragments without output skip their MFMAs
+    auto panel_part = [&](const ldsf* As, const ldsf* Bs) {
+#pragma unroll
+        for (int kk = 0; kk < KP / 4; ++kk) {
+            double a[FM], b[FN];
+#pragma unroll
+            for (int i = 0; i < FM; ++i) a[i] = As[(4 * kk + lg) * LDA_S + wm * (TM / 2) + i * 16 + li];
+#pragma unroll
+            for (int j = 0; j < FN; ++j) b[j] = Bs[(4 * kk + lg) * LDB_S + wn * (TN / 4) + j * 16 + li];
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int j = 0; j < FN; ++j)
+                    if (i < nfm && j < nfn) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+        }
+    };
+    const int np = (K + KP - 1) / KP;
+    if (np > 0) {
+        fetch(0);
+        store(lds);
+        __syncthreads();
+    }
+    if (XTB_GEMM_SPLIT(TA, TB)) {
+        // one panel loop per fragment pattern, so the accumulators are
+        // not merged between the two code paths after every panel
+        if (full) {
+            for (int p = 0; p < np; ++p) {
+                ldsf* cur = lds + (p & 1) * STAGE;
+                if (p + 1 < np) fetch((p + 1) * KP);
+                panel_full(cur, cur + KP * LDA_S);
+                if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
+                __syncthreads();
             }
         } else {
             for (int p = 0; p < np; ++p) {
                 ldsf* cur = lds + (p & 1) * STAGE;
                 if (p + 1 < np) fetch((p + 1) * KP);
-                if (full) panel_full(cur, cur + KP * LDA_S);
-                else panel_part(cur, cur + KP * LDA_S);
+                panel_part(cur, cur + KP * LDA_S);
                 if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
                 __syncthreads();
             }
         }
+    } else {
+        for (int p = 0; p < np; ++p) {
+            ldsf* cur = lds + (p & 1) * STAGE;
+            if (p + 1 < np) fetch((p + 1) * KP);
+            if (full) panel_full(cur, cur + KP * LDA_S);
+            else panel_part(cur, cur + KP * LDA_S);
+            if (p + 1 < np) store(lds + ((p + 1) & 1) * STAGE);
+            __syncthreads();
+        }
+    }
 #pragma unroll
-        for (int i = 0; i < FM; ++i)
+    for (int i = 0; i < FM; ++i)
 #pragma unroll
-            for (int j = 0; j < FN; ++j)
+        for (int j = 0; j < FN; ++j)
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const int gm = m0 + wm * (TM / 2) + i * 16 + lg + 4 * r;
-                    const int gn = n0 + wn * (TN / 4) + j * 16 + li;
-                    if (gm < M && gn < N) {
-                        const size_t e = (size_t)gm * ldc + gn;
-                        if (EPI) C[e] = fma(alpha, acc[i][j][r], fma(s1, E1[e], s2 * E2[e]));
-                        else C[e] = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * C[e]);
-                    }
+            for (int r = 0; r < 4; ++r) {
+                const int gm = m0 + wm * (TM / 2) + i * 16 + lg + 4 * r;
+                const int gn = n0 + wn * (TN / 4) + j * 16 + li;
+                if (gm < M && gn < N) {
+                    const size_t e = (size_t)gm * ldc + gn;
+                    if (EPI) C[e] = fma(alpha, acc[i][j][r], fma(s1, E1[e], s2 * E2[e]));
+                    else C[e] = (beta == 0.0) ? alpha * acc[i][j][r] : fma(alpha, acc[i][j][r], beta * C[e]);
                 }
+            }
+}
+
+// C = alpha op(A) op(B) + beta C on matrix cores, all tiles by one workgroup
+// (see gemm_tile).  With `upper` (M == N, a product known to be symmetric)
+// only the tiles on and above the diagonal are formed; gemm_sym mirrors the
+// rest.
+template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false>
+__device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
+                                          int lda, const glbf* __restrict__ B, int ldb, double beta,
+                                          glbf* __restrict__ C, int ldc, ldsf* lds, int upper,
+                                          const glbf* __restrict__ E1 = nullptr,
+                                          const glbf* __restrict__ E2 = nullptr, double s1 = 0.0,
+                                          double s2 = 0.0) {
+    constexpr int TM = 32 * FM, TN = 64 * FN;
+    // arguments of an out-of-line call arrive in vector registers; they are
+    // uniform, and saying so turns the tile and fragment tests into scalar
+    // branches instead of exec-masked code
+    M = __builtin_amdgcn_readfirstlane(M);
+    N = __builtin_amdgcn_readfirstlane(N);
+    K = __builtin_amdgcn_readfirstlane(K);
+    lda = __builtin_amdgcn_readfirstlane(lda);
+    ldb = __builtin_amdgcn_readfirstlane(ldb);
+    ldc = __builtin_amdgcn_readfirstlane(ldc);
+    upper = __builtin_amdgcn_readfirstlane(upper);
+    const int tm = (M + TM - 1) / TM, tn = (N + TN - 1) / TN;
+    for (int tile = 0; tile < tm * tn; ++tile) {
+        if (upper && tile / tn > tile % tn) continue;
+        gemm_tile<TA, TB, FM, FN, KP, EPI>((tile / tn) * TM, (tile % tn) * TN, M, N, K, alpha, A, lda, B,
+                                           ldb, beta, C, ldc, lds, E1, E2, s1, s2);
     }
     __syncthreads();
 }

@@ -296,7 +296,15 @@ class HostArena:
 class Pipeline:
     """Device buffers for one batch and the calls into the HIP library."""
 
-    def __init__(self, layout: BatchLayout, dtables: DeviceTables, device, arena: HostArena | None = None):
+    def __init__(self, layout: BatchLayout, dtables: DeviceTables, device, arena: HostArena | None = None,
+                 large: bool = False):
+        """``large``: one molecule, whose SCF runs on the whole GPU
+        (xtb_scf_large: grid kernels, a host-side driver) instead of one
+        workgroup (xtb_scf)."""
+        if large and (layout.B != 1 or layout.padded):
+            raise ValueError("the whole-GPU path takes one molecule in the ragged layout")
+        self.large = large
+        self.work = None
         self.L = layout
         self.arena = arena
         self.dt = dtables
@@ -422,13 +430,25 @@ class Pipeline:
         if timing:
             ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             ev0.record(st)
-        _lib.check(lib.xtb_scf(b, p, self.S.data_ptr(), self.H0.data_ptr(),
-                               self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state), s),
-                   "xtb_scf")
+        if self.large:
+            if self.work is None:
+                nw = ct.c_int64(0)
+                _lib.check(lib.xtb_scf_large_workspace(b, ct.byref(nw)), "xtb_scf_large_workspace")
+                self.work = torch.empty(int(nw.value), dtype=torch.float64, device=self.device)
+            o.keep_state = 0
+            o.warm_start = 0
+            _lib.check(lib.xtb_scf_large(b, p, self.S.data_ptr(), self.H0.data_ptr(),
+                                         self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state),
+                                         self.work.data_ptr(), self.work.numel(), s),
+                       "xtb_scf_large")
+        else:
+            _lib.check(lib.xtb_scf(b, p, self.S.data_ptr(), self.H0.data_ptr(),
+                                   self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state), s),
+                       "xtb_scf")
         if timing:
             ev1.record(st)
             self.scf_events = (ev0, ev1)
-        self._kept = bool(opts.warm_start)
+        self._kept = bool(opts.warm_start) and not self.large
         self._have_w = bool(forces)
         if forces:
             self.run_forces(stream)

@@ -288,6 +288,30 @@ int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par,
                 const xtb_scf_opts* opts, const xtb_scf_state* state,
                 void* stream);
 
+/* Scratch (doubles) of xtb_scf_large for this one-molecule batch: one more
+ * n x n matrix, the block-Jacobi rotations, orderings, reduction partials
+ * and control values.  Host-only.  XTB_EINVAL unless nmol == 1;
+ * XTB_ETOOBIG above 8192 orbitals. */
+int32_t xtb_scf_large_workspace(const xtb_batch* batch, int64_t* doubles);
+
+/* xtb_scf for ONE large molecule (thousands of orbitals) on the whole GPU:
+ * the same SCF (orthogonaliser, Fock build, perturbative occupied-virtual
+ * rotation for gapped closed shells, block-Jacobi eigensolve otherwise,
+ * Fermi smearing, density, Mulliken charges, ES2/ES3 potential, Anderson
+ * mixing, energies, W) with every stage a grid kernel over all CUs and the
+ * SCF loop on the host (a few control values read back per step, so the
+ * call returns after the SCF has finished on `stream`).  Same state buffers
+ * (F, C, T, P: one n x n each, stored at offset 0) and result record as
+ * xtb_scf, so xtb_forces and xtb_param_grad follow unchanged; keep_state and
+ * warm_start are ignored.  `work` holds xtb_scf_large_workspace doubles.
+ * Replaces the same upstream call sites as xtb_scf
+ * (src/dxtb/_src/scf/iterator.py:51-144, scf/unrolling/default.py:71-401,
+ * with scf/unrolling/base.py:142-175 for the eigensolve). */
+int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par,
+                      const double* S, const double* H0, const double* gamma,
+                      const xtb_scf_opts* opts, const xtb_scf_state* state,
+                      double* work, int64_t work_doubles, void* stream);
+
 int32_t xtb_repulsion(const xtb_batch* batch, const xtb_params* par,
                       const double* positions, double* out, void* stream);
 

@@ -1,5 +1,9 @@
-"""Whole-GPU large-system path (purification on matrix-core GEMMs) vs the
-oracle and vs the per-molecule kernel on the same molecules."""
+"""Whole-GPU large-system path (xtb_scf_large: grid GEMMs, grid block
+Jacobi, device-side Fermi smearing, density, Mulliken and mixing) vs the
+oracle and vs the per-molecule kernel on the same molecules: gapped closed
+shells (perturbative steps), hot molecules with fractional occupations
+(block Jacobi in every step), charged and open-shell molecules, orbital
+energies, and the 600-water cluster of config 3."""
 
 import numpy as np
 import pytest
@@ -7,6 +11,7 @@ import torch
 
 import oracle.gfn1 as og
 from conftest import mol
+from dxtb_amd import _lib
 from dxtb_amd import workloads as W
 from dxtb_amd.engine import ScfOptions
 from dxtb_amd.large import LargeSystem
@@ -19,6 +24,15 @@ pytestmark = pytest.mark.gpu
 def _need_gpu():
     if not torch.cuda.is_available():
         pytest.skip("no HIP device")
+
+
+def _check(r, o, e_tol=1e-9, q_tol=1e-8, f_tol=1e-8):
+    assert r["converged"] and r["status"] == 0, r["status"]
+    assert abs(r["energy"] - o["energy"]) < e_tol, (r["energy"], o["energy"])
+    assert abs(r["free_energy"] - o["fenergy"]) < e_tol
+    assert np.abs(r["charges_orbital"].cpu().numpy() - o["charges_orbital"].numpy()).max() < q_tol
+    assert np.abs(r["charges_atom"].cpu().numpy() - o["charges_atom"].numpy()).max() < q_tol
+    assert np.abs(r["gradient"].cpu().numpy() - o["gradient"].numpy()).max() < f_tol
 
 
 @pytest.mark.parametrize("case", ["lys_xao", "water40", "chain30"])
@@ -34,12 +48,51 @@ def test_large_path_matches_oracle(fx, case):
     ls = LargeSystem(z, "cuda:0", opts=ScfOptions(**TIGHT))
     r = ls.run(p, forces=True)
     o = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
-    assert r["converged"]
-    assert abs(r["energy"] - o["energy"]) < 1e-9
-    assert np.abs(r["charges_orbital"].cpu().numpy() - o["charges_orbital"].numpy()).max() < 1e-8
-    assert np.abs(r["gradient"].cpu().numpy() - o["gradient"].numpy()).max() < 1e-8
+    _check(r, o)
+    # gapped closed shells: intermediate steps by the perturbative rotation
+    assert r["records"][_lib.OUT_PD_STEPS] > 0
     k = run_native([(z, p)], forces=True)
     assert abs(k.energies().item() - r["energy"]) < 1e-9
+    assert np.abs(k.gradient().cpu().numpy() - r["gradient"].cpu().numpy()).max() < 1e-9
+
+
+@pytest.mark.parametrize("etemp", [5000.0, 20000.0])
+def test_large_path_fractional_occupations(etemp):
+    """Hot water cluster: Fermi occupations away from 0/1 around the Fermi
+    level and a sizeable -TS; every step is a block-Jacobi eigensolve on the
+    whole GPU."""
+    z, p = W.water_cluster(np.random.default_rng(31), 30)
+    print(f"case: large path, water30 at {etemp} K", flush=True)
+    ls = LargeSystem(z, "cuda:0", opts=ScfOptions(**TIGHT, fermi_etemp=etemp))
+    r = ls.run(p, forces=True)
+    o = og.singlepoint(z, p, opts={**TIGHT_ORACLE, "fermi_etemp": etemp}, with_forces=True)
+    _check(r, o)
+    assert -o["fenergy"] > (1e-6 if etemp < 1e4 else 1e-4)
+    assert r["records"][_lib.OUT_PD_STEPS] == 0
+
+
+def test_large_path_charged_and_open_shell():
+    """A chain cation (odd electron count: doublet, as upstream picks it) and
+    the same chain as a triplet: alpha and beta channels of different size."""
+    z, p = W.chain(np.random.default_rng(32), 20, p_n=0.0, p_o=0.0)
+    for chrg, spin in ((1.0, None), (0.0, 2)):
+        print(f"case: large path, chain20 charge {chrg} spin {spin}", flush=True)
+        ls = LargeSystem(z, "cuda:0", opts=ScfOptions(**TIGHT), chrg=chrg, spin=spin)
+        r = ls.run(p, forces=True)
+        o = og.singlepoint(z, p, chrg=chrg, spin=spin, opts=TIGHT_ORACLE, with_forces=True)
+        _check(r, o)
+
+
+def test_large_path_orbital_energies():
+    """Orbital energies asked for: the final step diagonalises fully."""
+    z, p = W.water_cluster(np.random.default_rng(33), 20)
+    print("case: large path, orbital energies", flush=True)
+    ls = LargeSystem(z, "cuda:0", opts=ScfOptions(**TIGHT, orbital_energies=True))
+    r = ls.run(p, forces=True)
+    o = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
+    _check(r, o)
+    emo = ls.pipe.emo[: len(o["emo"])].cpu().numpy()
+    assert np.abs(emo - o["emo"].numpy()).max() < 1e-8
 
 
 def test_config3_water600_vs_oracle_fixture():
@@ -60,10 +113,14 @@ def test_config3_water600_vs_oracle_fixture():
                           opts={"x_atol": 1e-10, "x_atol_max": 1e-10, "maxiter": 300,
                                 "exclude": ["disp"]})
     e = calc.get_energy(pos)
+    assert calc._pipe.large
     assert abs(float(e) - float(ref["energy"])) < 1e-8
     f = calc.forces_analytical(pos)
     assert np.abs(-f.cpu().numpy() - ref["gradient"]).max() < 1e-7
     q = calc.get_charges(pos)
     assert q.shape == (4800,) and abs(float(q.sum())) < 1e-8
-    qat = calc._run(pos, forces=False)["large"]["charges_atom"]
-    assert np.abs(qat.cpu().numpy() - ref["charges_atom"]).max() < 1e-7
+    L = calc._pipe.L
+    ao_atom = L.arrays["sh_atom"][L.arrays["ao_sh"]].astype(np.int64)
+    qat = np.zeros(1800)
+    np.add.at(qat, ao_atom, q.cpu().numpy())
+    assert np.abs(qat - ref["charges_atom"]).max() < 1e-7

@@ -91,34 +91,23 @@ def test_sto_expansion_identical_to_oracle():
         assert np.allclose(a, ao.numpy(), rtol=1e-15, atol=0) and np.allclose(c, co.numpy(), rtol=1e-15, atol=0)
 
 
-def _sym_with_spectrum(ev, seed):
-    g = torch.Generator().manual_seed(seed)
-    n = len(ev)
-    Q, _ = torch.linalg.qr(torch.randn(n, n, generator=g, dtype=torch.float64))
-    return Q @ torch.diag(torch.as_tensor(ev, dtype=torch.float64)) @ Q.T
-
-
-def test_purification_projector_and_gap_estimates():
-    """Whole-GPU path helpers (dxtb_amd/large.py) on the CPU: TC2 gives the
-    spectral projector onto the nocc lowest eigenvectors, and the Lanczos
-    estimates of the frontier eigenvalues bound them from inside.  The second
-    case has few occupied states (the Krylov space of the projected operator
-    runs out long before the step limit)."""
-    from dxtb_amd.large import lanczos_extreme, tc2
-
-    for n, nocc, seed in ((200, 70, 0), (94, 8, 1)):
-        ev = np.sort(np.random.default_rng(seed).normal(size=n))
-        ev[nocc:] += 0.4
-        F = _sym_with_spectrum(ev, seed)
-        D = tc2(F, nocc)
-        w, V = torch.linalg.eigh(F)
-        P = V[:, :nocc] @ V[:, :nocc].T
-        assert float((D - P).abs().max()) < 1e-12
-        eye = torch.eye(n, dtype=torch.float64)
-        homo = lanczos_extreme(F, D, largest=True)
-        lumo = lanczos_extreme(F, eye - D, largest=False)
-        assert homo <= float(w[nocc - 1]) + 1e-12 and homo > float(w[nocc - 1]) - 1e-4
-        assert lumo >= float(w[nocc]) - 1e-12 and lumo < float(w[nocc]) + 1e-4
+def test_large_path_workspace_query():
+    """xtb_scf_large_workspace (host-only): one molecule, the size of its
+    scratch grows like n^2, and orbital counts beyond the sort's limit or
+    batches of several molecules are refused before any device work."""
+    lib = L.load()
+    out = ct.c_int64(0)
+    sizes = []
+    for n in (700, 1400, 4800):
+        b = L.XtbBatch(nmol=1, max_n=n, max_nsh=n // 2, max_nat=n // 3)
+        assert lib.xtb_scf_large_workspace(ct.byref(b), ct.byref(out)) == L.XTB_OK
+        assert out.value >= n * n + 4 * n
+        sizes.append(out.value)
+    assert 3.5 < sizes[1] / sizes[0] < 4.5
+    b = L.XtbBatch(nmol=1, max_n=8193, max_nsh=4000, max_nat=3000)
+    assert lib.xtb_scf_large_workspace(ct.byref(b), ct.byref(out)) == L.XTB_ETOOBIG
+    b = L.XtbBatch(nmol=2, max_n=800, max_nsh=400, max_nat=300)
+    assert lib.xtb_scf_large_workspace(ct.byref(b), ct.byref(out)) == L.XTB_EINVAL
 
 
 def test_warm_start_flags():
