@@ -75,7 +75,7 @@ class XtbScfOpts(ct.Structure):
         ("diag_offset", f64), ("fermi_thresh", f64), ("jacobi_tol", f64), ("eig_inexact", f64),
         ("bd_gap_kt", f64), ("pd_kmax", f64), ("first_tol", f64), ("ortho", i32), ("orbitals", i32),
         ("keep_state", i32), ("warm_start", i32), ("soft_start", i32),
-        ("wg_skip", i32),
+        ("wg_skip", i32), ("mixer", i32), ("f_atol", f64),
     ]
 
 

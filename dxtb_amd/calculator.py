@@ -15,11 +15,13 @@ Differences from upstream, by design:
 * D3 dispersion needs the reference C6 tables of tad-dftd3, which are not
   part of this package; the dispersion term is left out (with a warning
   unless "disp" is in ``opts["exclude"]``).
-* The SCF always mixes the potential with the Anderson mixer (upstream's
-  default full mode falls back to Anderson as well,
-  src/dxtb/_src/scf/unrolling/base.py:76-99); ``scf_mode`` and ``mixer``
-  are accepted for compatibility and change neither the fixed point nor the
-  result.
+* ``scf_mode`` selects the mixer as upstream does: the full (unrolling)
+  mode mixes the potential with Anderson (a "broyden" mixer falls back to
+  Anderson there, src/dxtb/_src/scf/unrolling/base.py:76-99); the implicit
+  modes ("implicit", "default", "nonpure", ...) run xitorch's broyden1 root
+  search (src/dxtb/_src/scf/pure/iterator.py:125-147).  Both reach the same
+  fixed point; forces are analytic in either mode.  A limited Broyden
+  memory (24 updates, then a restart) stands in for upstream's unlimited one.
 """
 
 from __future__ import annotations
@@ -71,7 +73,41 @@ def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:
         raise NotImplementedError("damp_dynamic is not implemented")
     if not 1 <= int(o.generations) <= 6:
         raise ValueError("damp_generations must be in 1..6")
+    if "broyden_memory" in rest:
+        o.broyden_memory = int(rest.pop("broyden_memory"))
+        if not 1 <= o.broyden_memory <= 24:
+            raise ValueError("broyden_memory must be in 1..24")
     return o, rest
+
+
+# upstream's SCF mode names and codes (src/dxtb/_src/constants/labels/scf.py)
+_FULL = ("full", "full_tracking", "unrolling", "0")
+_IMPLICIT = ("default", "implicit", "1", "implicit_old", "implicit_nonpure", "nonpure", "non-pure",
+             "old", "2")
+_MIXERS = {"anderson": "anderson", "a": "anderson", "1": "anderson",
+           "broyden": "broyden", "b": "broyden", "2": "broyden",
+           "linear": "linear", "l": "linear", "simple": "linear", "s": "linear", "0": "linear"}
+
+
+def _scf_mode(scf_mode, mixer) -> str:
+    """The mixer of the device SCF for upstream's scf_mode / mixer options:
+    implicit modes always run broyden1 (scf/pure/iterator.py:125-147,
+    scf/implicit/default.py: the mixer option is ignored); the full mode
+    runs the configured mixer, with "broyden" falling back to Anderson
+    (scf/unrolling/base.py:76-99)."""
+    m = str(scf_mode).lower()
+    mix = _MIXERS.get(str(mixer).lower())
+    if mix is None:
+        raise ValueError(f"Unknown mixer '{mixer}'.")
+    if m in _IMPLICIT:
+        return "broyden"
+    if m in _FULL:
+        if mix == "linear":
+            raise NotImplementedError("the simple (linear) mixer is not implemented; use 'anderson'")
+        return "anderson"
+    if m in ("experimental", "perfect", "single-shot", "single_shot", "3"):
+        raise NotImplementedError(f"scf_mode '{scf_mode}' is not implemented")
+    raise ValueError(f"Unknown SCF mode '{scf_mode}'.")
 
 
 class _EnergyFn(torch.autograd.Function):
@@ -117,11 +153,16 @@ class GFN1Calculator:
         self.exclude = {exclude} if isinstance(exclude, str) else set(exclude)
         # upstream's result cache is off by default (constants/defaults.py:266)
         self.cache_enabled = bool(opts.pop("cache_enabled", False))
-        for k in ("scf_mode", "mixer", "int_driver", "int_level", "int_uplo", "batch_mode",
-                  "method", "strict", "guess", "f_atol", "fermi_partition",
+        mode = _scf_mode(opts.pop("scf_mode", "full"), opts.pop("mixer", "broyden"))
+        f_atol = opts.pop("f_atol", None)
+        for k in ("int_driver", "int_level", "int_uplo", "batch_mode",
+                  "method", "strict", "guess", "fermi_partition",
                   "max_element", "anomaly", "grad", "skip_compat_checks", "int_cutoff"):
             opts.pop(k, None)
         self.scf_opts, rest = _opts_to_scf(opts)
+        self.scf_opts.mixer = mode
+        if f_atol is not None:
+            self.scf_opts.f_atol = float(f_atol)
         self.scf_opts.jacobi_max_sweeps = int(kwargs.pop("jacobi_max_sweeps", 40))
         self.layout_padded = bool(kwargs.pop("padded", True))
         if rest:

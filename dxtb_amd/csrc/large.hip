@@ -398,7 +398,8 @@ __global__ void __launch_bounds__(NT) k_japply(double* A_, double* Ct_, int n, i
 // ---------------------------------------------------------------------------
 enum {
     C_EFA = 0, C_EFB, C_FOK, C_KOCC, C_PERM, C_GAPA, C_GAPB, C_DMAX,
-    C_AOV, C_KMAX, C_DMIN, C_XF, C_OFF, C_ERR, C_GERSH, C_CONV, C_RESID, C_FSOFT, C_NCTL = 24
+    C_AOV, C_KMAX, C_DMIN, C_XF, C_OFF, C_ERR, C_GERSH, C_CONV, C_RESID, C_FSOFT, C_BRANK, C_BDX,
+    C_NCTL = 24
 };
 
 struct SharedG {
@@ -554,11 +555,12 @@ __global__ void __launch_bounds__(256) k_spread(int n, int nao, const int* ao_sh
     v[i] = (s >= 0) ? vsh[s] : 0.0;
 }
 
-// Anderson mixing (one workgroup; the same mixer as the per-molecule
-// kernel): ctl[C_CONV] = converged, ctl[C_RESID] = |x_new - x_old|_inf
-__global__ void __launch_bounds__(NT) k_anderson(Mixer mx, int n, xtb_scf_opts o, double* hx, double* hf,
-                                                 const double* xnew, const double* xold, double* xout,
-                                                 double* ctl) {
+// Mixing (one workgroup; the same mixers as the per-molecule kernel):
+// Anderson, or Broyden when o.mixer == 1.  ctl[C_CONV] = converged,
+// ctl[C_RESID] = the residual's max norm; the Broyden memory's rank and last
+// step norm go to ctl[C_BRANK], ctl[C_BDX] for the host's copy of its state
+__global__ void __launch_bounds__(NT) k_mix(Mixer mx, Broyden bm, int n, xtb_scf_opts o, double* hx, double* hf,
+                                            const double* xnew, const double* xold, double* xout, double* ctl) {
     __shared__ double red[256];
     struct { ldsf* red; } sh{to_lds(red)};
     Mol M;
@@ -566,10 +568,14 @@ __global__ void __launch_bounds__(NT) k_anderson(Mixer mx, int n, xtb_scf_opts o
     M.vec0 = 0;
     M.nvec_total = n;
     double resid = 0.0;
-    const bool c = anderson(mx, M, sh, o, to_glb(hx), to_glb(hf), to_glb(xnew), to_glb(xold), to_glb(xout), &resid);
+    const bool c = o.mixer == 1
+        ? broyden(bm, M, sh, o, to_glb(hx), to_glb(hf), to_glb(xnew), to_glb(xold), to_glb(xout), &resid)
+        : anderson(mx, M, sh, o, to_glb(hx), to_glb(hf), to_glb(xnew), to_glb(xold), to_glb(xout), &resid);
     if (threadIdx.x == 0) {
         ctl[C_CONV] = c ? 1.0 : 0.0;
         ctl[C_RESID] = resid;
+        ctl[C_BRANK] = (double)bm.rank;
+        ctl[C_BDX] = bm.dxnorm;
     }
 }
 
@@ -1051,14 +1057,20 @@ int step(Ctx& c, const double* vin, double* vout, int& status, double loose, boo
     return sw < 0 ? 0 : sw;
 }
 
-bool mix(Ctx& c, Mixer& mx, const double* xnew, const double* xold, double* xout, double* resid) {
-    hipLaunchKernelGGL(k_anderson, dim3(1), dim3(NT), 0, c.s, mx, c.n, c.o, c.st->hist_x, c.st->hist_f, xnew, xold,
+bool mix(Ctx& c, Mixer& mx, Broyden& bm, const double* xnew, const double* xold, double* xout, double* resid) {
+    hipLaunchKernelGGL(k_mix, dim3(1), dim3(NT), 0, c.s, mx, bm, c.n, c.o, c.st->hist_x, c.st->hist_f, xnew, xold,
                        xout, c.ctl);
     chk_launch(c);
-    // the kernel's own copy of the mixer advanced the same way
-    mx.step += 1;
-    mx.head = (mx.head - 1 + mx.G) % mx.G;
     if (!read_ctl(c)) return false;
+    // the kernel's own copy of the mixer advanced the same way
+    if (c.o.mixer == 1) {
+        bm.step += 1;
+        bm.rank = (int)c.h[C_BRANK];
+        bm.dxnorm = c.h[C_BDX];
+    } else {
+        mx.step += 1;
+        mx.head = (mx.head - 1 + mx.G) % mx.G;
+    }
     *resid = c.h[C_RESID];
     return c.h[C_CONV] != 0.0;
 }
@@ -1078,7 +1090,9 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
     if (!batch || !par || !S || !H0 || !gamma || !opts || !state || !work) return XTB_EINVAL;
     if (batch->nmol != 1 || batch->max_n <= 0 || batch->has_padding) return XTB_EINVAL;
     if (batch->max_n > LARGE_MAX_N) return XTB_ETOOBIG;
-    if (opts->generations < 1 || opts->generations > MIX_MAXG) return XTB_EINVAL;
+    if (opts->mixer != 0 && opts->mixer != 1) return XTB_EINVAL;
+    if (opts->generations < 1 || opts->generations > (opts->mixer == 1 ? BROYDEN_MAXR : MIX_MAXG))
+        return XTB_EINVAL;
     if (opts->jacobi_inner < 1 || opts->jacobi_inner > 20 || opts->eig_inexact < 0.0) return XTB_EINVAL;
     if (!state->F || !state->C || !state->T || !state->P || !state->pot || !state->vnew || !state->q ||
         !state->eorb || !state->emo || !state->occ || !state->hist_x || !state->hist_f || !state->qsh ||
@@ -1160,14 +1174,16 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
     chk(c, hipMemsetAsync(state->pot, 0, sizeof(double) * n, c.s));
     const xtb_scf_opts& o = c.o;
     Mixer mx{0, 0, o.generations + 1, o.soft_start != 0};
+    Broyden bm{0, 0, o.generations, 0.0};
+    const bool use_broyden = o.mixer == 1;
     int iters = 0, sweeps = 0;
     double resid = 1e-2;
     const double loose0 = o.eig_inexact > 0.0 ? o.first_tol : 0.0;
     const int sw_first = step(c, state->pot, state->vnew, status, loose0, false, true, false, nullptr, false);
     sweeps += sw_first;
     ++iters;
-    if (o.mix_guess) {
-        mix(c, mx, state->vnew, state->pot, state->pot, &resid);
+    if (o.mix_guess || use_broyden) {
+        mix(c, mx, bm, state->vnew, state->pot, state->pot, &resid);
     } else {
         chk(c, hipMemcpyAsync(state->pot, state->vnew, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
     }
@@ -1179,10 +1195,14 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
         sweeps += step(c, state->pot, state->vnew, status, o.eig_inexact * resid * soft, o.eig_inexact > 0.0,
                        true, false, nullptr, false);
         ++iters;
-        if (mix(c, mx, state->vnew, state->pot, state->pot, &resid)) {
+        if (mix(c, mx, bm, state->vnew, state->pot, state->pot, &resid)) {
             converged = true;
             break;
         }
+    }
+    if (use_broyden) {
+        // the final density comes from the root search's result (in pot)
+        chk(c, hipMemcpyAsync(state->vnew, state->pot, sizeof(double) * n, hipMemcpyDeviceToDevice, c.s));
     }
     if (!converged && o.maxiter > 0) status |= XTB_MOL_MAXITER;
     int kdec = 0;

@@ -892,6 +892,17 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         __syncthreads();
     }
     Mixer mx{0, 0, o.generations + 1, !warm && o.soft_start != 0};
+    Broyden bm{0, 0, o.generations, 0.0};
+    const bool use_broyden = o.mixer == 1;
+    // one mixing step after a map evaluation v_in = pot -> vnew; the mixed
+    // potential goes to pot.  Broyden, on convergence, leaves in pot the
+    // input its root search returns (the one before the last step,
+    // rootsolver.py:200-215)
+    auto mix = [&](double* resid_out) -> bool {
+        if (!use_broyden)
+            return anderson(mx, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, resid_out);
+        return broyden(bm, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, resid_out);
+    };
     int iters = 0;
     // unrolling/default.py:80-100: evaluate the guess, optionally mix it
     // Inexact SCF: each step's eigensolve is converged only as far as the
@@ -905,8 +916,8 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
                                   warm && o.eig_inexact > 0.0, true);
     sweeps += sw_first;
     ++iters;
-    if (o.mix_guess) {
-        anderson(mx, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, &resid);
+    if (o.mix_guess || use_broyden) {
+        mix(&resid);
     } else {
         for (int i = threadIdx.x; i < M.n; i += NT) M.pot[i] = M.vnew[i];
         __syncthreads();
@@ -919,8 +930,13 @@ __global__ void __launch_bounds__(NT) k_scf(xtb_batch b, xtb_params par, const d
         sweeps += scf_step(M, sh, b, par, o, M.pot, M.vnew, &status, ef, o.eig_inexact * resid * soft,
                            o.eig_inexact > 0.0, true);
         ++iters;
-        const bool c = anderson(mx, M, sh, o, to_glb(st.hist_x), to_glb(st.hist_f), M.vnew, M.pot, M.pot, &resid);
-        if (c) { converged = true; break; }
+        if (mix(&resid)) { converged = true; break; }
+    }
+    if (use_broyden) {
+        // the final density comes from the root search's result (in pot):
+        // the final step reads its input from vnew
+        for (int i = threadIdx.x; i < M.n; i += NT) M.vnew[i] = M.pot[i];
+        __syncthreads();
     }
     if (!converged && o.maxiter > 0) status |= XTB_MOL_MAXITER;
     if (o.maxiter == 0) converged = true;
@@ -1062,7 +1078,9 @@ extern "C" int32_t xtb_scf(const xtb_batch* batch, const xtb_params* par, const 
     if (!batch || !par || !S || !H0 || !gamma || !opts || !state) return XTB_EINVAL;
     if (batch->nmol <= 0 || batch->max_n <= 0) return XTB_EINVAL;
     if (batch->max_n > 1024) return XTB_ETOOBIG;
-    if (opts->generations < 1 || opts->generations > MIX_MAXG) return XTB_EINVAL;
+    if (opts->mixer != 0 && opts->mixer != 1) return XTB_EINVAL;
+    if (opts->generations < 1 || opts->generations > (opts->mixer == 1 ? BROYDEN_MAXR : MIX_MAXG))
+        return XTB_EINVAL;
     if (opts->wg_skip < 0 || opts->wg_skip > 7) return XTB_EINVAL;
     if (opts->jacobi_inner < 1 || opts->jacobi_inner > 20 || opts->eig_inexact < 0.0) return XTB_EINVAL;
     if (!state->F || !state->C || !state->T || !state->P || !state->pot || !state->vnew ||

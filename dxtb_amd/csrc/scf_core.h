@@ -253,4 +253,128 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const SH& sh, con
     return conv;
 }
 
+// ---------------------------------------------------------------------------
+// Limited-memory Broyden root search ("broyden1", good Broyden) on
+// g(x) = x - f(x), as upstream's implicit SCF modes run it
+// (scf/implicit/default.py, scf/pure/iterator.py:125-147 -> xitorch
+// equilibrium, optimize/root/rootsolver.py:42-256 with BroydenFirst,
+// _jacobian.py:50-112): inverse Jacobian H = h0 I + sum_k c_k d_k^T with
+// h0 = -alpha = 0.5, no line search, x <- x + damp dx with dx = -H g.  After
+// each map evaluation (x_prev, g_prev -> x, g): dx = x - x_prev,
+// dy = g - g_prev, v = H^T dx, c = dx - H dy, d = v / (dy.v), H += c d^T.
+// Converged (TerminationCondition, rootsolver.py:471-492) when the last
+// step's |dx| < x_atol and |g| < f_atol.  Upstream keeps every update; here
+// at most R = generations (<= BROYDEN_MAXR) are kept and the memory restarts
+// from h0 I when full (upstream's "restart" reduction at max_rank = R).
+// Storage: c_k in hx slot k, d_k in hf slot k (k < R); x_prev, g_prev in
+// slot R.  The first call only records x_prev and g_prev.
+// ---------------------------------------------------------------------------
+constexpr int BROYDEN_MAXR = 24;
+
+struct Broyden {
+    int step, rank, R;
+    double dxnorm;  // |dx| of the last step (before damping)
+};
+
+// Projections sum_i U_k[i] w_i for k < r, reduced in chunks of 8 and left
+// in LDS at red[128 + k] (red holds 256 doubles; wg_sum_n<8> uses the first
+// 64).  U_k(i) and w(i) are callables.
+template <class SH, class UK, class W>
+__device__ inline void wg_projections(int r, int n, const SH& sh, UK U, W w, int out0) {
+    for (int k0 = 0; k0 < r; k0 += 8) {
+        double t[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+        for (int i = threadIdx.x; i < n; i += NT) {
+            const double wi = w(i);
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+                if (k0 + kk < r) t[kk] = fma(U(k0 + kk, i), wi, t[kk]);
+        }
+        wg_sum_n<8>(t, sh.red);
+        if (threadIdx.x == 0) {
+#pragma unroll
+            for (int kk = 0; kk < 8; ++kk)
+                if (k0 + kk < r) sh.red[128 + out0 + k0 + kk] = t[kk];
+        }
+        __syncthreads();
+    }
+}
+
+template <class SH>
+__device__ __noinline__ bool broyden(Broyden& bm, const Mol& M, const SH& sh, const xtb_scf_opts& o,
+                                     glbf* hx, glbf* hf, const glbf* fx, const glbf* x, glbf* xout,
+                                     double* resid_inf) {
+    static_assert(128 + 2 * BROYDEN_MAXR <= 256, "projections fit the reduction scratch");
+    const int n = M.n, R = bm.R;
+    constexpr double h0 = 0.5;  // -alpha, alpha = -0.5 (scf/pure/iterator.py:132)
+    auto C = [&](int k) { return hx + (size_t)k * M.nvec_total + M.vec0; };
+    auto D = [&](int k) { return hf + (size_t)k * M.nvec_total + M.vec0; };
+    glbf* xp = C(R);
+    glbf* gp = D(R);
+    ldsf* proj = sh.red + 128;  // [0, R): first set, [R, 2R): second set
+    double g2 = 0.0, ginf = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double g = x[i] - fx[i];
+        g2 = fma(g, g, g2);
+        ginf = fmax(ginf, fabs(g));
+    }
+    g2 = wg_sum(g2, sh.red);
+    ginf = wg_max(ginf, sh.red);
+    *resid_inf = ginf;
+    if (bm.step > 0) {
+        if (bm.dxnorm < o.x_atol && sqrt(g2) < o.f_atol) {
+            // the root search stops before it takes x: its result is the
+            // previous input (rootsolver.py:200-215 breaks before x = xnew)
+            for (int i = threadIdx.x; i < n; i += NT) xout[i] = xp[i];
+            __syncthreads();
+            bm.step += 1;
+            return true;
+        }
+        // c_k.dx and d_k.dy over the kept pairs; the new pair goes to slot rr
+        // (the memory restarts when full)
+        const int r = bm.rank;
+        const int rr = r >= R ? 0 : r;
+        wg_projections(rr, n, sh, [&](int k, int i) { return C(k)[i]; },
+                       [&](int i) { return x[i] - xp[i]; }, 0);
+        wg_projections(rr, n, sh, [&](int k, int i) { return D(k)[i]; },
+                       [&](int i) { return (x[i] - fx[i]) - gp[i]; }, R);
+        glbf* cn = C(rr);
+        glbf* dn = D(rr);
+        double dyv = 0.0;
+        for (int i = threadIdx.x; i < n; i += NT) {
+            const double dx = x[i] - xp[i], dy = (x[i] - fx[i]) - gp[i];
+            double v = h0 * dx, hdy = h0 * dy;
+            for (int k = 0; k < rr; ++k) {
+                v = fma(D(k)[i], proj[k], v);
+                hdy = fma(C(k)[i], proj[R + k], hdy);
+            }
+            cn[i] = dx - hdy;
+            dn[i] = v;
+            dyv = fma(dy, v, dyv);
+        }
+        dyv = wg_sum(dyv, sh.red);
+        const double sc = 1.0 / dyv;
+        for (int i = threadIdx.x; i < n; i += NT) dn[i] *= sc;
+        __syncthreads();
+        bm.rank = rr + 1;
+    } else {
+        bm.rank = 0;
+    }
+    // dx = -H g with the updated memory
+    const int rk = bm.rank;
+    wg_projections(rk, n, sh, [&](int k, int i) { return D(k)[i]; }, [&](int i) { return x[i] - fx[i]; }, 0);
+    double dn2 = 0.0;
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const double xi = x[i], g = xi - fx[i];
+        double hg = h0 * g;
+        for (int k = 0; k < rk; ++k) hg = fma(C(k)[i], proj[k], hg);
+        // x_prev, g_prev <- x, g
+        xp[i] = xi;
+        gp[i] = g;
+        dn2 = fma(hg, hg, dn2);
+        xout[i] = xi - o.damp * hg;  // xout may alias x: element-wise read-before-write
+    }
+    bm.dxnorm = sqrt(wg_sum(dn2, sh.red));
+    bm.step += 1;
+    return false;
+}
 }  // namespace xtb

@@ -238,14 +238,28 @@ class ScfOptions:
     # `generations` steps before Anderson acceleration (warm starts always
     # skip it); off, Anderson starts at the second step
     soft_start: bool = False
+    # "anderson" (upstream's full mode) or "broyden" (upstream's implicit
+    # modes: xitorch's broyden1 root search on v - f(v), converged when the
+    # last step's |dv| < x_atol and |v - f(v)| < f_atol)
+    mixer: str = "anderson"
+    f_atol: float = 1e-4
+    # Broyden memory (updates kept before it restarts; upstream keeps all)
+    broyden_memory: int = 24
+
+    def history_slots(self) -> int:
+        """Vectors per molecule in each of the two mixer history buffers."""
+        return (self.broyden_memory if self.mixer == "broyden" else self.generations) + 1
 
     def to_struct(self, want_w: bool, padded: bool, warm: bool = False, wg_skip: int = 0) -> _lib.XtbScfOpts:
+        if self.mixer not in ("anderson", "broyden"):
+            raise ValueError(f"Unknown mixer '{self.mixer}'.")
         if not self.fermi_etemp >= 0.0:
             # filling.py:320-324
             raise ValueError(f"Electronic Temperature must be positive or None ({self.fermi_etemp}).")
         thr = self.fermi_thresh if self.fermi_thresh is not None else float(np.finfo(np.float64).eps) ** 0.5
         return _lib.XtbScfOpts(
-            maxiter=self.maxiter, mix_guess=int(self.mix_guess), generations=self.generations,
+            maxiter=self.maxiter, mix_guess=int(self.mix_guess),
+            generations=self.history_slots() - 1,
             fermi_maxiter=self.fermi_maxiter, want_w=int(want_w),
             jacobi_max_sweeps=self.jacobi_max_sweeps, padded=int(padded),
             jacobi_inner=self.jacobi_inner, eig_inexact=self.eig_inexact,
@@ -255,7 +269,7 @@ class ScfOptions:
             pd_kmax=self.pd_kmax, first_tol=self.first_tol, ortho=self.ortho,
             orbitals=int(self.orbital_energies), keep_state=int(self.warm_start),
             warm_start=int(self.warm_start and warm), soft_start=int(self.soft_start),
-            wg_skip=int(wg_skip),
+            wg_skip=int(wg_skip), mixer=int(self.mixer == "broyden"), f_atol=self.f_atol,
         )
 
 
@@ -398,9 +412,9 @@ class Pipeline:
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         s = ct.c_void_p(st.cuda_stream)
         lib, b, p = self.lib, ct.byref(self.batch), ct.byref(self.dt.s)
-        g = opts.generations
-        if self.hist_x is None or self.hist_x.numel() != (g + 1) * self.L.nvec_total:
-            self.hist_x = torch.empty((g + 1) * self.L.nvec_total, dtype=torch.float64, device=self.device)
+        g = opts.history_slots()
+        if self.hist_x is None or self.hist_x.numel() != g * self.L.nvec_total:
+            self.hist_x = torch.empty(g * self.L.nvec_total, dtype=torch.float64, device=self.device)
             self.hist_f = torch.empty_like(self.hist_x)
         pos = positions.data_ptr()
         if timing:

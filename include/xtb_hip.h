@@ -229,6 +229,13 @@ typedef struct xtb_scf_opts {
                                 mostly starts each launch at XCD 0, so launches
                                 of a few molecules pile up on the first XCDs;
                                 a serving loop rotates this to spread them */
+    int32_t mixer;           /* 0: Anderson (upstream's full mode, mixer/anderson.py);
+                                1: limited-memory Broyden root search on
+                                x - f(x) (upstream's implicit modes: xitorch
+                                broyden1, alpha -0.5, no line search); its
+                                memory holds `generations` (1..24) updates */
+    double f_atol;           /* Broyden: converged when the last step's |dx| <
+                                x_atol and |x - f(x)| < f_atol */
 } xtb_scf_opts;
 
 /* Work and result buffers of xtb_scf (device, caller-owned). */

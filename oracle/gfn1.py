@@ -523,8 +523,12 @@ def repulsion(numbers, positions, par: dict | None = None, grad=False):
 # Fermi smearing (a22): _src/wavefunction/filling.py:41-366
 # ---------------------------------------------------------------------------
 def fermi_occupation(nel_channel: float, emo: torch.Tensor, kt: float,
-                     maxiter: int = 200, thr: float | None = None) -> torch.Tensor:
-    """Fermi occupations of one spin channel (filling.py:201-366, unpadded)."""
+                     maxiter: int = 200, thr: float | None = None, nao: int | None = None) -> torch.Tensor:
+    """Fermi occupations of one spin channel (filling.py:201-366).  With
+    `nao` the levels beyond index nao of the sorted `emo` count as a padded
+    batch's padding slots for the start value: the Fermi level starts at
+    twice the HOMO/LUMO midpoint when either index falls on one
+    (filling.py:254-264, the orbital mask gathered by index)."""
     eps = torch.finfo(DTYPE).eps
     if abs(nel_channel) < eps:
         return torch.zeros_like(emo)
@@ -537,6 +541,8 @@ def fermi_occupation(nel_channel: float, emo: torch.Tensor, kt: float,
     else:
         ef = emo[homo].clone()
     ef = torch.as_tensor(ef, dtype=DTYPE)
+    if nao is not None and (homo >= nao or min(homo + 1, n - 1) >= nao):
+        ef = 2.0 * ef
     target = homo + 1
     for _ in range(maxiter):
         x = (emo - ef) / kt
@@ -628,6 +634,59 @@ class Anderson:
         return bool(torch.linalg.norm(d) < x_tol and d.abs().max() < x_tol_max)
 
 
+class Broyden:
+    """Root search on g(x) = x - f(x) as upstream's implicit SCF modes run it
+    (scf/pure/iterator.py:125-147 -> xitorch equilibrium,
+    optimize/rootfinder.py:145-235; optimize/root/rootsolver.py:42-256 with
+    line_search False; BroydenFirst, _jacobian.py:50-112; LowRankMatrix,
+    _jacobian.py:250-292): inverse Jacobian H = -alpha I + sum c d^T with
+    alpha = -0.5, dx = -H g, x <- x + damp dx; after each evaluation
+    v = H^T dx, c = dx - H dy, d = v / (dy.v); stop when |dx| < x_tol and
+    |g| < f_tol, returning the input before the last step.  `max_rank`
+    restarts the memory when more updates would be kept (upstream: none)."""
+
+    def __init__(self, damp=0.5, alpha=-0.5, max_rank=None):
+        self.damp, self.h0, self.max_rank = damp, -alpha, max_rank
+
+    def _mv(self, v):
+        r = self.h0 * v
+        for c, d in zip(self.cs, self.ds):
+            r = r + c * torch.dot(d, v)
+        return r
+
+    def _rmv(self, v):
+        r = self.h0 * v
+        for c, d in zip(self.cs, self.ds):
+            r = r + d * torch.dot(c, v)
+        return r
+
+    def solve(self, func, x0, maxiter, f_tol, x_tol):
+        """Returns (x, converged)."""
+        self.cs, self.ds = [], []
+        x = x0.clone()
+        y = x - func(x)
+        if float(y.norm()) == 0.0:
+            return x, True
+        xp, yp = x, y
+        for _ in range(maxiter):
+            dx = -self._mv(y)
+            xnew = x + self.damp * dx
+            ynew = xnew - func(xnew)
+            # update (BroydenFirst.update; the memory is reduced first)
+            if self.max_rank is not None and len(self.cs) >= self.max_rank:
+                self.cs, self.ds = [], []
+            ddx, ddy = xnew - xp, ynew - yp
+            v = self._rmv(ddx)
+            c = ddx - self._mv(ddy)
+            self.cs.append(c)
+            self.ds.append(v / torch.dot(ddy, v))
+            xp, yp = xnew, ynew
+            if float(dx.norm()) < x_tol and float(ynew.norm()) < f_tol:
+                return x, True
+            x, y = xnew, ynew
+        return x, False
+
+
 def eigh_generalized(F, S):
     """Cholesky reduction + symmetric eigensolve (tad_mctc storch.eighb as
     called at _src/scf/unrolling/base.py:141-175; xitorch exacteig,
@@ -648,9 +707,10 @@ DEFAULT_OPTS = {
 
 
 class _SCF:
-    def __init__(self, bas, par, S, H0, gamma, hd, n0, nel_ab, kt, opts):
+    def __init__(self, bas, par, S, H0, gamma, hd, n0, nel_ab, kt, opts, npad=0):
         self.bas, self.S, self.H0, self.gamma, self.hd = bas, S, H0, gamma, hd
         self.n0, self.nel_ab, self.kt, self.opts = n0, nel_ab, kt, opts
+        self.npad = int(npad)
         self.ao_sh = torch.tensor(bas.ao_shell)
         self.sh_at = torch.tensor(bas.shell_atom)
         self.iter = 0
@@ -672,18 +732,38 @@ class _SCF:
         F = self.H0 - 0.5 * self.S * (v[:, None] + v[None, :])  # base.py:817-859
         self.fock = F
         e, C = eigh_generalized(F, self.S)
+        # a molecule inside a padded batch (npad > 0): the padding slots of
+        # the generalised eigenproblem (zero Fock rows, unit overlap
+        # diagonal) are levels at exactly 0 that take part in the filling
+        # and in -TS (filling.py:269-366 masks only the start value)
+        npad = self.npad
+        n = e.numel()
+        if npad:
+            e_all = torch.cat([e, torch.zeros(npad, dtype=DTYPE)])
+            order = torch.argsort(e_all, stable=True)
+            e_fill = e_all[order]
+        else:
+            e_fill = e
         if self.kt < KT_AUFBAU:
             # base.py:888-889: no smearing; the aufbau occupations of the
             # (unrounded) alpha/beta counts set up by get_refocc stay
             # (iterator.py:147-189)
-            self.occ = torch.stack([aufbau_occupation(e.numel(), nel) for nel in self.nel_ab])
+            occ = torch.stack([aufbau_occupation(e_fill.numel(), nel) for nel in self.nel_ab])
         else:
             occ = []
             for nel in self.nel_ab:
-                occ.append(fermi_occupation(round(nel), e, self.kt,
+                occ.append(fermi_occupation(round(nel), e_fill, self.kt,
                                             self.opts["fermi_maxiter"],
-                                            self.opts["fermi_thresh"]))
-            self.occ = torch.stack(occ)
+                                            self.opts["fermi_thresh"],
+                                            nao=n if npad else None))
+            occ = torch.stack(occ)
+        if npad:
+            full = torch.empty_like(occ)
+            full[:, order] = occ
+            self.occ_all = full
+            self.occ = full[:, :n]
+        else:
+            self.occ_all = self.occ = occ
         self.emo, self.C = e, C
         f = self.occ.sum(0)
         return (C * f) @ C.T  # scf/utils.py:28-52
@@ -707,11 +787,14 @@ def _n0(bas: Basis):
 
 def singlepoint(numbers, positions, chrg: float = 0.0, spin=None,
                 opts: dict | None = None, par: dict | None = None,
-                with_forces: bool = False) -> dict:
+                with_forces: bool = False, npad: int = 0) -> dict:
     """Energy (and optionally analytic forces) of ONE molecule.
 
     Follows _src/calculators/types/energy.py:78-402 with dispersion and
     halogen excluded (halogen is zero for H/C/N/O, hal.py:307-309).
+    `npad` > 0: the molecule as one entry of upstream's padded batch whose
+    orbital dimension exceeds its own by npad (padding levels at energy 0
+    share the Fermi filling and -TS).
     """
     par = par or load_params()
     o = dict(DEFAULT_OPTS)
@@ -732,21 +815,28 @@ def singlepoint(numbers, positions, chrg: float = 0.0, spin=None,
         # filling.py:320-324
         raise ValueError(f"Electronic Temperature must be positive or None ({o['fermi_etemp']}).")
     kt = o["fermi_etemp"] * KELVIN2AU
-    scf = _SCF(bas, par, S, H0, gam, hd, n0, (na, nb), kt, o)
+    scf = _SCF(bas, par, S, H0, gam, hd, n0, (na, nb), kt, o, npad=npad)
     mixer = Anderson(o["damp"], o["damp_init"], o["generations"], o["damp_diagonal_offset"])
 
-    # unrolling/default.py:71-140 (single-system branch)
     guess = scf.charges_to_potential(torch.zeros(bas.nao, dtype=DTYPE))
-    q_new = scf.iterate(guess)
-    q = mixer.iter(q_new, guess) if o["mix_guess"] else q_new
-    converged = False
-    for _ in range(o["maxiter"]):
-        q_new = scf.iterate(q)
-        q = mixer.iter(q_new, q)
-        if mixer.converged(o["x_atol"], o["x_atol_max"]):
-            converged = True
-            break
-    v_conv = q_new
+    if str(o.get("scf_mode", "full")).lower() in ("implicit", "default", "nonpure"):
+        # implicit modes: broyden1 on v - f(v); the density of the result
+        # (scf/pure/iterator.py:75-97)
+        br = Broyden(damp=o["damp"], max_rank=o.get("broyden_memory"))
+        v_conv, converged = br.solve(scf.iterate, guess, o["maxiter"], o.get("f_atol", 1e-4),
+                                     o["x_atol"])
+    else:
+        # unrolling/default.py:71-140 (single-system branch)
+        q_new = scf.iterate(guess)
+        q = mixer.iter(q_new, guess) if o["mix_guess"] else q_new
+        converged = False
+        for _ in range(o["maxiter"]):
+            q_new = scf.iterate(q)
+            q = mixer.iter(q_new, q)
+            if mixer.converged(o["x_atol"], o["x_atol_max"]):
+                converged = True
+                break
+        v_conv = q_new
     # converged_to_charges (base.py:472-513): one more density build
     P = scf.potential_to_density(v_conv)
     qorb = scf.density_to_charges(P)
@@ -758,8 +848,8 @@ def singlepoint(numbers, positions, chrg: float = 0.0, spin=None,
     e_scf_atom = e_h0 + e_es2 + e_es3
     # electronic free energy, equal partition (base.py:544-623)
     eps = torch.finfo(DTYPE).eps
-    occ = torch.clamp(scf.occ, min=eps)
-    occ1 = torch.clamp(1 - scf.occ, min=eps)
+    occ = torch.clamp(scf.occ_all, min=eps)
+    occ1 = torch.clamp(1 - scf.occ_all, min=eps)
     g = torch.log(occ ** occ * occ1 ** occ1).sum() * kt
     e_free_atom = torch.full((bas.nat,), float(g) / bas.nat, dtype=DTYPE)
     e_rep_atom = repulsion(numbers, positions, par)

@@ -214,3 +214,59 @@ def test_batched_overlap_equals_pair_loop(fx):
     S2, d2 = og._overlap_impl_loop(bas, pos, True)
     assert (S1 - S2).abs().max() < 1e-15
     assert (d1 - d2).abs().max() < 1e-15
+
+
+def test_broyden_updates_are_good_broyden():
+    """The oracle's broyden1 (upstream's implicit-mode root search) keeps the
+    inverse of Broyden's FIRST ("good") Jacobian update: with B = H^-1,
+    B_new = B + (dy - B dx) dx^T / (dx^T dx) after every step, and H_new
+    satisfies the secant condition H_new dy = dx.  The second ("bad")
+    method would satisfy the secant condition too but fails the first
+    identity.  On an affine contraction the search returns the fixed point."""
+    rng = np.random.default_rng(3)
+    n = 6
+    A = torch.as_tensor(0.3 * rng.standard_normal((n, n)), dtype=torch.float64)
+    b = torch.as_tensor(rng.standard_normal(n), dtype=torch.float64)
+    func = lambda x: A @ x + b
+    br = og.Broyden(damp=0.5)
+    dense = lambda: br.h0 * torch.eye(n, dtype=torch.float64) + sum(
+        (torch.outer(c, d) for c, d in zip(br.cs, br.ds)), torch.zeros(n, n, dtype=torch.float64))
+    # replay the search step by step, checking each update
+    br.cs, br.ds = [], []
+    x = torch.zeros(n, dtype=torch.float64)
+    y = x - func(x)
+    for _ in range(4):
+        H = dense()
+        dx_step = -(H @ y)
+        xn = x + 0.5 * dx_step
+        yn = xn - func(xn)
+        ddx, ddy = xn - x, yn - y
+        v = H.T @ ddx
+        br.cs.append(ddx - H @ ddy)
+        br.ds.append(v / torch.dot(ddy, v))
+        Hn = dense()
+        assert torch.allclose(Hn @ ddy, ddx, atol=1e-12)
+        B, Bn = torch.linalg.inv(H), torch.linalg.inv(Hn)
+        good = B + torch.outer(ddy - B @ ddx, ddx) / torch.dot(ddx, ddx)
+        assert torch.allclose(Bn, good, atol=1e-10)
+        x, y = xn, yn
+    xs, ok = og.Broyden(damp=0.5).solve(func, torch.zeros(n, dtype=torch.float64), 200, 1e-12, 1e-12)
+    exact = torch.linalg.solve(torch.eye(n, dtype=torch.float64) - A, b)
+    assert ok and float((xs - exact).abs().max()) < 1e-11
+
+
+def test_implicit_mode_reaches_the_full_mode_fixed_point():
+    """Upstream's implicit mode (broyden1 on v - f(v)) and full mode
+    (Anderson) share the SCF fixed point; the root search takes more
+    steps.  A restart of the Broyden memory changes only the path."""
+    from dxtb_amd import workloads as W
+
+    z, p = W.chain(np.random.default_rng(3), 12)
+    o = {"x_atol": 1e-10, "x_atol_max": 1e-10, "maxiter": 300}
+    a = og.singlepoint(z, p, opts=o)
+    b = og.singlepoint(z, p, opts={**o, "scf_mode": "implicit", "f_atol": 1e-10})
+    c = og.singlepoint(z, p, opts={**o, "scf_mode": "implicit", "f_atol": 1e-10, "broyden_memory": 8})
+    assert a["converged"] and b["converged"] and c["converged"]
+    assert abs(a["energy"] - b["energy"]) < 1e-10 and abs(b["energy"] - c["energy"]) < 1e-10
+    assert float((a["charges_orbital"] - b["charges_orbital"]).abs().max()) < 1e-9
+    assert b["iterations"] != a["iterations"]
