@@ -34,6 +34,18 @@ Pinning status (what checks each function; tests/test_oracle.py):
 * Aufbau filling at zero temperature: upstream's doctest values
   (wavefunction/filling.py:135-147) and the SiH4 0 K occupations.
 * alpha/beta split and its errors: upstream's rules (filling.py:41-113).
+* Implicit-mode root search (Broyden, xitorch broyden1): each update is
+  checked to be the inverse of Broyden's first ("good") Jacobian update and
+  to satisfy the secant condition; it returns the exact fixed point of an
+  affine contraction and the full-mode (Anderson) SCF fixed point.  Its
+  step-by-step path (iteration counts) has no upstream vector: parity of
+  the path is unpinned beyond those identities.
+* Padded-batch filling (npad padding levels at energy 0 sharing the Fermi
+  filling and -TS): follows filling.py:201-366, where only the start value
+  is masked; upstream's batched filling tests
+  (test/test_wavefunction/test_filling.py:125-185) pack zero-energy padding
+  into the same filling.  No upstream vector holds a padded result at high
+  temperature: parity unpinned beyond the unpadded limit (npad = 0).
 * Repulsion: test/test_classical/test_repulsion/samples.py (energies and
   the H2O gradient).
 * Analytic forces: tblite SCF-only gradients (test/test_scf/grad.npz,
