@@ -40,6 +40,13 @@ namespace big {
 // `upper` only tiles on and above the diagonal are formed (a product known
 // to be symmetric; k_mirror fills the rest).
 // ---------------------------------------------------------------------------
+// control values the host reads back (slots of the `ctl` array)
+enum {
+    C_EFA = 0, C_EFB, C_FOK, C_KOCC, C_PERM, C_GAPA, C_GAPB, C_DMAX,
+    C_AOV, C_KMAX, C_DMIN, C_XF, C_OFF, C_ERR, C_GERSH, C_CONV, C_RESID, C_FSOFT, C_BRANK, C_BDX,
+    C_NCTL = 24
+};
+
 constexpr int GT = 128;
 constexpr int GEMM_LDS_BYTES = 2 * GEMM_KP * (GT + 1 + GT + 1) * 8;
 
@@ -287,6 +294,52 @@ __global__ void __launch_bounds__(NT) k_dev_identity(int n, const double* T, dou
 }
 
 // ---------------------------------------------------------------------------
+// Cholesky orthogonaliser X = L^-1, S = L L^T (the per-workgroup version is
+// chol_setup in scf.hip), 32-wide panels.  k_chol_diag: one wave factors the
+// diagonal block j0 of the lower triangle of L in place and writes its
+// inverse to Dg (nb x nb, leading dimension nb); a non-positive pivot sets
+// ctl[C_ERR].  With factor = 0 it only inverts the (already factored) block.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(WAVE) k_chol_diag(double* L_, int n, int j0, int factor, double* Dg_,
+                                                    double* ctl) {
+    constexpr int NB = 32;
+    __shared__ double dl[NB * NB], dil[NB * NB];
+    ldsf* D = to_lds(dl);
+    ldsf* Di = to_lds(dil);
+    glbf* L = to_glb(L_);
+    const int lane = threadIdx.x;
+    const int nb = n - j0 < NB ? n - j0 : NB;
+    for (int e = lane; e < NB * NB; e += WAVE) {
+        const int r = e / NB, c = e % NB;
+        D[e] = (r < nb && c < nb) ? ((factor || c <= r) ? L[(size_t)(j0 + r) * n + j0 + c] : 0.0)
+                                  : (r == c ? 1.0 : 0.0);
+    }
+    wave_sync();
+    if (factor) {
+        const bool ok = chol_block_factor(D, lane);
+        if (!ok && lane == 0) ctl[C_ERR] = 1.0;
+        for (int e = lane; e < nb * nb; e += WAVE) {
+            const int r = e / nb, c = e % nb;
+            if (c <= r) L[(size_t)(j0 + r) * n + j0 + c] = D[r * NB + c];
+        }
+        wave_sync();
+    }
+    chol_block_inverse(D, Di, to_glb(Dg_), nb, lane);
+}
+
+// Y[r][c] = X[r][c] for r < rows, c < cols (leading dimensions ldx, ldy)
+__global__ void __launch_bounds__(256) k_copy_block(int rows, int cols, const double* X, int ldx, double* Y,
+                                                    int ldy) {
+    const int r = blockIdx.y;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256)
+        Y[(size_t)r * ldy + c] = X[(size_t)r * ldx + c];
+}
+
+__global__ void k_set_slot(double* ctl, int slot, double v) {
+    if (threadIdx.x == 0) ctl[slot] = v;
+}
+
+// ---------------------------------------------------------------------------
 // Block Jacobi on the whole GPU (the per-workgroup version is
 // wg_block_jacobi).  Round r of the round-robin order pairs the 16-index
 // blocks; k_jsub solves each pair's 32 x 32 subproblem with one wave
@@ -396,11 +449,6 @@ __global__ void __launch_bounds__(NT) k_japply(double* A_, double* Ct_, int n, i
 //   ctl[C_DMAX]              max |ev|
 //   ctl[C_FSOFT]             largest f (1 - f) of either channel
 // ---------------------------------------------------------------------------
-enum {
-    C_EFA = 0, C_EFB, C_FOK, C_KOCC, C_PERM, C_GAPA, C_GAPB, C_DMAX,
-    C_AOV, C_KMAX, C_DMIN, C_XF, C_OFF, C_ERR, C_GERSH, C_CONV, C_RESID, C_FSOFT, C_BRANK, C_BDX,
-    C_NCTL = 24
-};
 
 struct SharedG {
     ldsf *skey, *red;
@@ -826,6 +874,56 @@ bool ortho(Ctx& c) {
     return it < 60;
 }
 
+// Cholesky orthogonaliser (as chol_setup in scf.hip): S = L L^T by
+// right-looking 32-wide panels (a wave factors each diagonal block, the panel
+// solve and the trailing update are grid GEMMs), then X = L^-1 by the
+// right-looking triangular inverse: block row j of X <- D_j^-1 X_j, rows
+// below -= L_(>j, j) X_j, on the identity.  L in T, X in the orbital buffer,
+// R (rows x 32, leading dimension n) in F, the block inverse in P.  Returns
+// false when S is not numerically positive definite.
+bool ortho_chol(Ctx& c) {
+    constexpr int NB = 32;
+    const int n = c.n;
+    double* L = c.T;
+    double* X = c.Ct;
+    double* R = c.F;
+    double* Dg = c.P;
+    chk(c, hipMemcpyAsync(L, c.S, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c.s));
+    hipLaunchKernelGGL(k_set_slot, dim3(1), dim3(1), 0, c.s, c.ctl, (int)C_ERR, 0.0);
+    chk_launch(c);
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nb = n - j0 < NB ? n - j0 : NB;
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(WAVE), 0, c.s, L, n, j0, 1, Dg, c.ctl);
+        chk_launch(c);
+        const int mt = n - j0 - nb;
+        if (mt > 0) {
+            double* Pn = L + (size_t)(j0 + nb) * n + j0;  // panel below the block
+            gemm_t<false, true, false>(c, mt, nb, nb, 1.0, Pn, n, Dg, nb, 0.0, R, n, false);  // b D^-T
+            hipLaunchKernelGGL(k_copy_block, dim3(1, mt), dim3(256), 0, c.s, mt, nb, R, n, Pn, n);
+            chk_launch(c);
+            gemm_t<false, true, false>(c, mt, mt, nb, -1.0, Pn, n, Pn, n, 1.0, Pn + nb, n, false);
+        }
+    }
+    if (!read_ctl(c) || c.h[C_ERR] != 0.0) return false;
+    hipLaunchKernelGGL(k_identity, dim3((n + 255) / 256, n), dim3(256), 0, c.s, n, X);
+    chk_launch(c);
+    for (int j0 = 0; j0 < n; j0 += NB) {
+        const int nb = n - j0 < NB ? n - j0 : NB;
+        const int w = j0 + nb;  // non-zero columns of block row j
+        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(WAVE), 0, c.s, L, n, j0, 0, Dg, c.ctl);
+        chk_launch(c);
+        double* Xj = X + (size_t)j0 * n;
+        gemm_t<false, false, false>(c, nb, w, nb, 1.0, Dg, nb, Xj, n, 0.0, R, n, false);  // D^-1 X_j
+        hipLaunchKernelGGL(k_copy_block, dim3((w + 255) / 256, nb), dim3(256), 0, c.s, nb, w, R, n, Xj, n);
+        chk_launch(c);
+        const int mt = n - w;
+        if (mt > 0)
+            gemm_t<false, false, false>(c, mt, w, nb, -1.0, L + (size_t)w * n + j0, n, Xj, n, 1.0,
+                                        X + (size_t)w * n, n, false);
+    }
+    return c.err == hipSuccess;
+}
+
 // Block Jacobi on A = F (full symmetric); rotations applied to the orbital
 // rows.  Tolerance as eig_step in scf.hip.  Eigenvalue estimates -> ev.
 // Returns sweeps, or -1 when max_sweeps were spent.
@@ -1157,7 +1255,7 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
         rec.v[slot] = v;
         rec.mask |= 1u << slot;
     };
-    const bool ok_ortho = ortho(c);
+    const bool ok_ortho = opts->ortho == 1 ? ortho_chol(c) : ortho(c);
     const double t_ortho = now_s() - tstart;
     if (c.err != hipSuccess) return XTB_EHIP;
     if (!ok_ortho) {

@@ -178,28 +178,6 @@ __device__ __noinline__ int ortho_setup(const Mol& M, const Shared& sh, int* sta
 // Returns 0; a non-positive pivot (S not positive definite, where upstream's
 // torch.linalg.cholesky raises) sets XTB_MOL_OVERLAP in *status.
 // ---------------------------------------------------------------------------
-__device__ void chol_block_inverse(ldsf* D, ldsf* Di, glbf* Dg, int nb, int lane) {
-    // lower-triangular D (NB x NB, LDS) -> Di = D^-1 (LDS) and its leading
-    // nb x nb block to Dg (global, leading dimension nb: it must fit in an
-    // n x n buffer when n < NB); lane j forms column j by forward substitution
-    constexpr int NB = 32;
-    if (lane < NB) {
-        const int j = lane;
-        for (int i = 0; i < NB; ++i) {
-            double v;
-            if (i < j) v = 0.0;
-            else if (i == j) v = 1.0 / D[j * NB + j];
-            else {
-                double acc = 0.0;
-                for (int k = j; k < i; ++k) acc = fma(D[i * NB + k], Di[k * NB + j], acc);
-                v = -acc / D[i * NB + i];
-            }
-            Di[i * NB + j] = v;
-            if (i < nb && j < nb) Dg[i * nb + j] = v;
-        }
-    }
-}
-
 // In-place blocked right-looking Cholesky L L^T = A of the n x n symmetric
 // positive definite A (lower triangle read and overwritten by L; the strict
 // upper triangle is left as it was), 32-wide panels: one wave factors and
@@ -223,21 +201,7 @@ __device__ __noinline__ bool chol_factor(glbf* L, int n, glbf* R, glbf* Dg, cons
         }
         __syncthreads();
         if (w == 0) {
-            for (int c = 0; c < NB; ++c) {
-                const double d = D[c * NB + c];
-                wave_sync();
-                const double sd = d > 0.0 ? sqrt(d) : 1.0;
-                if (!(d > 0.0)) bad = true;
-                if (lane == 0) D[c * NB + c] = sd;
-                for (int r = c + 1 + lane; r < NB; r += WAVE) D[r * NB + c] /= sd;
-                wave_sync();
-                const int m = NB - c - 1;
-                for (int e = lane; e < m * m; e += WAVE) {
-                    const int rr = c + 1 + e / m, qq = c + 1 + e % m;
-                    if (qq <= rr) D[rr * NB + qq] -= D[rr * NB + c] * D[qq * NB + c];
-                }
-                wave_sync();
-            }
+            if (!chol_block_factor(D, lane)) bad = true;
             chol_block_inverse(D, Di, Dg, nb, lane);
         }
         __syncthreads();

@@ -38,6 +38,55 @@ struct Shared {
 // device wall clock (constant 100 MHz) in microseconds
 __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 
+// ---------------------------------------------------------------------------
+// Diagonal blocks of the blocked Cholesky orthogonaliser (one wave, 32 x 32
+// in LDS, row-major).  chol_block_factor: D <- its lower Cholesky factor in
+// place (the upper triangle is left stale); returns false when a pivot is
+// not positive (the pivot is then taken as 1 and the factor is meaningless).
+// ---------------------------------------------------------------------------
+__device__ inline bool chol_block_factor(ldsf* D, int lane) {
+    constexpr int NB = 32;
+    bool ok = true;
+    for (int c = 0; c < NB; ++c) {
+        const double d = D[c * NB + c];
+        wave_sync();
+        const double sd = d > 0.0 ? sqrt(d) : 1.0;
+        if (!(d > 0.0)) ok = false;
+        if (lane == 0) D[c * NB + c] = sd;
+        for (int r = c + 1 + lane; r < NB; r += WAVE) D[r * NB + c] /= sd;
+        wave_sync();
+        const int m = NB - c - 1;
+        for (int e = lane; e < m * m; e += WAVE) {
+            const int rr = c + 1 + e / m, qq = c + 1 + e % m;
+            if (qq <= rr) D[rr * NB + qq] -= D[rr * NB + c] * D[qq * NB + c];
+        }
+        wave_sync();
+    }
+    return ok;
+}
+
+__device__ inline void chol_block_inverse(ldsf* D, ldsf* Di, glbf* Dg, int nb, int lane) {
+    // lower-triangular D (NB x NB, LDS) -> Di = D^-1 (LDS) and its leading
+    // nb x nb block to Dg (global, leading dimension nb: it must fit in an
+    // n x n buffer when n < NB); lane j forms column j by forward substitution
+    constexpr int NB = 32;
+    if (lane < NB) {
+        const int j = lane;
+        for (int i = 0; i < NB; ++i) {
+            double v;
+            if (i < j) v = 0.0;
+            else if (i == j) v = 1.0 / D[j * NB + j];
+            else {
+                double acc = 0.0;
+                for (int k = j; k < i; ++k) acc = fma(D[i * NB + k], Di[k * NB + j], acc);
+                v = -acc / D[i * NB + i];
+            }
+            Di[i * NB + j] = v;
+            if (i < nb && j < nb) Dg[i * nb + j] = v;
+        }
+    }
+}
+
 // electronic temperatures below this (0.1 K in Hartree) fill the orbitals
 // by the aufbau principle instead of Fermi smearing
 // (src/dxtb/_src/scf/base.py:888-889)

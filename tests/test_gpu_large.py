@@ -35,17 +35,18 @@ def _check(r, o, e_tol=1e-9, q_tol=1e-8, f_tol=1e-8):
     assert np.abs(r["gradient"].cpu().numpy() - o["gradient"].numpy()).max() < f_tol
 
 
-@pytest.mark.parametrize("case", ["lys_xao", "water40", "chain30"])
+@pytest.mark.parametrize("case", ["lys_xao", "water40", "chain30", "water40-loewdin"])
 def test_large_path_matches_oracle(fx, case):
+    """Cholesky orthogonaliser (default), and Loewdin's by Newton-Schulz."""
     rng = np.random.default_rng(8)
     if case == "lys_xao":
         z, p = mol(fx, "lys_xao")
-    elif case == "water40":
+    elif case.startswith("water40"):
         z, p = W.water_cluster(rng, 40)
     else:
         z, p = W.chain(rng, 30)
     print(f"case: large path {case}", flush=True)
-    ls = LargeSystem(z, "cuda:0", opts=ScfOptions(**TIGHT))
+    ls = LargeSystem(z, "cuda:0", opts=ScfOptions(**TIGHT, ortho=0 if case.endswith("loewdin") else 1))
     r = ls.run(p, forces=True)
     o = og.singlepoint(z, p, opts=TIGHT_ORACLE, with_forces=True)
     _check(r, o)
