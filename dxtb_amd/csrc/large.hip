@@ -48,7 +48,13 @@ enum {
 };
 
 constexpr int GT = 128;
-constexpr int GEMM_LDS_BYTES = 2 * GEMM_KP * (GT + 1 + GT + 1) * 8;
+// k-panel depth of the grid GEMM (one workgroup per CU either way: its
+// register use allows three waves per SIMD, fewer than a second workgroup)
+#ifndef XTB_GRID_KP
+#define XTB_GRID_KP 16
+#endif
+constexpr int GRID_KP = XTB_GRID_KP;
+constexpr int GEMM_LDS_BYTES = 2 * GRID_KP * (GT + 1 + GT + 1) * 8;
 
 __device__ inline void tile_of(int idx, int tn, int upper, int& ti, int& tj) {
     if (!upper) {
@@ -77,7 +83,7 @@ __global__ void __launch_bounds__(NT) k_gemm(int M, int N, int K, double alpha, 
     const int tn = (N + GT - 1) / GT;
     int ti, tj;
     tile_of(idx, tn, upper, ti, tj);
-    gemm_tile<TA, TB, 4, 2, GEMM_KP, EPI>(ti * GT, tj * GT, M, N, K, alpha, to_glb(A), lda, to_glb(B), ldb,
+    gemm_tile<TA, TB, 4, 2, GRID_KP, EPI>(ti * GT, tj * GT, M, N, K, alpha, to_glb(A), lda, to_glb(B), ldb,
                                           beta, to_glb(C), ldc, to_lds(lds), to_glb(E1), to_glb(E2), s1, s2);
 }
 
