@@ -332,14 +332,35 @@ __device__ __noinline__ int gather_occupied(const Mol& M, const Shared& sh, glbf
 // ---------------------------------------------------------------------------
 // Fock build F = H0 - 1/2 S o (v_i + v_j) (scf/base.py:817-859)
 // ---------------------------------------------------------------------------
+// Streaming passes over rows run four rows per wave at a time, so that each
+// lane has the loads of four rows in flight (one row per wave left a CU's
+// waves waiting on memory latency at a fraction of its share of HBM);
+// rows past the end are clamped to the last row for the loads and skipped
+// for the stores.
+constexpr int ROWS_PER_WAVE = 4;
+
 __device__ inline void fock_build(const Mol& M, const glbf* vin) {
     const int n = M.n;
     const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
-    for (int r = w; r < n; r += NWAVE) {
-        const double vr = vin[r];
+    for (int r0 = w * ROWS_PER_WAVE; r0 < n; r0 += NWAVE * ROWS_PER_WAVE) {
+        int rr[ROWS_PER_WAVE];
+        double vr[ROWS_PER_WAVE];
+#pragma unroll
+        for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+            rr[u] = r0 + u < n ? r0 + u : n - 1;
+            vr[u] = vin[rr[u]];
+        }
         for (int c = lane; c < n; c += WAVE) {
-            const size_t e = (size_t)r * n + c;
-            M.F[e] = M.H0[e] - 0.5 * M.S[e] * (vr + vin[c]);
+            const double vc = vin[c];
+            double h[ROWS_PER_WAVE], sv[ROWS_PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                h[u] = M.H0[(size_t)rr[u] * n + c];
+                sv[u] = M.S[(size_t)rr[u] * n + c];
+            }
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u)
+                if (r0 + u < n) M.F[(size_t)rr[u] * n + c] = h[u] - 0.5 * sv[u] * (vr[u] + vc);
         }
     }
     __syncthreads();
@@ -403,11 +424,29 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         PD_MARK(sh, 0, tm);
         gemm<false, true>(k, m, n, 1.0, M.T, n, Cv, n, 0.0, Kb, m, sh.big);
         PD_MARK(sh, 1, tm);
-        for (int i = w; i < rows; i += NWAVE) {
-            double d = 0.0;
-            for (int c = lane; c < n; c += WAVE) d = fma(M.T[(size_t)i * n + c], Ct[(size_t)i * n + c], d);
-            d = wave_sum(d);
-            if (lane == 0) sh.ev[i] = d;
+        for (int i0 = w * ROWS_PER_WAVE; i0 < rows; i0 += NWAVE * ROWS_PER_WAVE) {
+            int ii[ROWS_PER_WAVE];
+            double d[ROWS_PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                ii[u] = i0 + u < rows ? i0 + u : rows - 1;
+                d[u] = 0.0;
+            }
+            for (int c = lane; c < n; c += WAVE) {
+                double tv[ROWS_PER_WAVE], cv[ROWS_PER_WAVE];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                    tv[u] = M.T[(size_t)ii[u] * n + c];
+                    cv[u] = Ct[(size_t)ii[u] * n + c];
+                }
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) d[u] = fma(tv[u], cv[u], d[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                const double v = wave_sum(d[u]);
+                if (lane == 0 && i0 + u < rows) sh.ev[i0 + u] = v;
+            }
         }
         for (int a = rows + threadIdx.x; a < n; a += NT) sh.ev[a] = sh.skey[a];
         __syncthreads();
@@ -416,16 +455,29 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         dmax = wg_max(dmax, sh.red);
         const double tol = fmax(o.jacobi_tol * fmax(dmax, 1.0), loose);
         double aov = 0.0, kmax = 0.0, dmin = INFINITY;
-        for (int i = w; i < k; i += NWAVE) {
-            const double ei = sh.ev[i];
+        for (int i0 = w * ROWS_PER_WAVE; i0 < k; i0 += NWAVE * ROWS_PER_WAVE) {
+            int ii[ROWS_PER_WAVE];
+            double ei[ROWS_PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                ii[u] = i0 + u < k ? i0 + u : k - 1;
+                ei[u] = sh.ev[ii[u]];
+            }
             for (int a = lane; a < m; a += WAVE) {
-                const double v = Kb[(size_t)i * m + a];
-                const double den = sh.ev[k + a] - ei;
-                const double amp = v / den;
-                aov = fmax(aov, fabs(v));
-                dmin = fmin(dmin, den);
-                kmax = fmax(kmax, fabs(amp));
-                Kb[(size_t)i * m + a] = amp;
+                const double ea = sh.ev[k + a];
+                double v[ROWS_PER_WAVE];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) v[u] = Kb[(size_t)ii[u] * m + a];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                    if (i0 + u >= k) continue;
+                    const double den = ea - ei[u];
+                    const double amp = v[u] / den;
+                    aov = fmax(aov, fabs(v[u]));
+                    dmin = fmin(dmin, den);
+                    kmax = fmax(kmax, fabs(amp));
+                    Kb[(size_t)ii[u] * m + a] = amp;
+                }
             }
         }
         aov = wg_max(aov, sh.red);
@@ -473,11 +525,7 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         glbf* cur = M.F;
         glbf* nxt = M.F + (size_t)k * n;
         // R = sinc(X) C_occ + phi(X) Z
-        for (int i = w; i < k; i += NWAVE)
-            for (int c = lane; c < n; c += WAVE) {
-                const size_t e = (size_t)i * n + c;
-                cur[e] = ss[J] * Ct[e] + ph[J] * Z[e];
-            }
+        wg_axpby((size_t)k * n, ss[J], Ct, ph[J], Z, cur);
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
@@ -492,11 +540,7 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         gemm<true, false>(m, n, k, 1.0, Kb, m, cur, n, 1.0, Cv, n, sh.big);
         PD_MARK(sh, 7, tm);
         // C_occ <- cos(X) C_occ - sinc(X) Z
-        for (int i = w; i < k; i += NWAVE)
-            for (int c = lane; c < n; c += WAVE) {
-                const size_t e = (size_t)i * n + c;
-                cur[e] = cc[J] * Ct[e] - ss[J] * Z[e];
-            }
+        wg_axpby((size_t)k * n, cc[J], Ct, -ss[J], Z, cur);
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
@@ -506,8 +550,7 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
             cur = nxt;
             nxt = t;
         }
-        for (int i = w; i < k; i += NWAVE)
-            for (int c = lane; c < n; c += WAVE) Ct[(size_t)i * n + c] = cur[(size_t)i * n + c];
+        wg_axpby((size_t)k * n, 1.0, cur, 0.0, nullptr, Ct);
         __syncthreads();
         PD_MARK(sh, 6, tm);
 #ifdef XTB_PD_TIMERS
@@ -644,9 +687,22 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         for (int j = threadIdx.x; j < n; j += NT)
             if (sh.ftot[j] > 0.0) k = j + 1;
         kocc = (int)wg_max((double)k, sh.red);
-        for (int j = w; j < kocc; j += NWAVE) {
-            const double f = sh.ftot[j];
-            for (int c = lane; c < n; c += WAVE) M.T[(size_t)j * n + c] = f * M.Ct[(size_t)j * n + c];
+        for (int j0 = w * ROWS_PER_WAVE; j0 < kocc; j0 += NWAVE * ROWS_PER_WAVE) {
+            int jj[ROWS_PER_WAVE];
+            double f[ROWS_PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                jj[u] = j0 + u < kocc ? j0 + u : kocc - 1;
+                f[u] = sh.ftot[jj[u]];
+            }
+            for (int c = lane; c < n; c += WAVE) {
+                double cv[ROWS_PER_WAVE];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) cv[u] = M.Ct[(size_t)jj[u] * n + c];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u)
+                    if (j0 + u < kocc) M.T[(size_t)jj[u] * n + c] = f[u] * cv[u];
+            }
         }
         __syncthreads();
     }
@@ -656,8 +712,7 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
 #endif
     if (sort_rows && perm) {
         // the next step starts from the orbitals in ascending energy order
-        for (int r = w; r < n; r += NWAVE)
-            for (int c = lane; c < n; c += WAVE) M.Ct[(size_t)r * n + c] = M.F[(size_t)r * n + c];
+        wg_axpby((size_t)n * n, 1.0, M.F, 0.0, nullptr, M.Ct);
     }
     sh.tacc[1] += now_us() - t0;
     const double tm0 = now_us();
@@ -665,26 +720,50 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     // per row; the orbital energies only enter the final energy (last step)
     {
         const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
-        for (int i = w; i < n; i += NWAVE) {
-            double sq = 0.0, se = 0.0;
-            if (i < M.nao) {
-                const glbf* pr = M.P + (size_t)i * n;
-                const glbf* sr = M.S + (size_t)i * n;
-                const glbf* hr = M.H0 + (size_t)i * n;
-                if (last) {
-                    for (int j = l; j < M.nao; j += WAVE) {
-                        sq = fma(pr[j], sr[j], sq);
-                        se = fma(pr[j], hr[j], se);
+        for (int i0 = w * ROWS_PER_WAVE; i0 < n; i0 += NWAVE * ROWS_PER_WAVE) {
+            int ii[ROWS_PER_WAVE];
+            double sq[ROWS_PER_WAVE], se[ROWS_PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                ii[u] = i0 + u < n ? i0 + u : n - 1;
+                sq[u] = 0.0;
+                se[u] = 0.0;
+            }
+            if (last) {
+                for (int j = l; j < M.nao; j += WAVE) {
+                    double pv[ROWS_PER_WAVE], sv[ROWS_PER_WAVE], hv[ROWS_PER_WAVE];
+#pragma unroll
+                    for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                        pv[u] = M.P[(size_t)ii[u] * n + j];
+                        sv[u] = M.S[(size_t)ii[u] * n + j];
+                        hv[u] = M.H0[(size_t)ii[u] * n + j];
                     }
-                } else {
-                    for (int j = l; j < M.nao; j += WAVE) sq = fma(pr[j], sr[j], sq);
+#pragma unroll
+                    for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                        sq[u] = fma(pv[u], sv[u], sq[u]);
+                        se[u] = fma(pv[u], hv[u], se[u]);
+                    }
+                }
+            } else {
+                for (int j = l; j < M.nao; j += WAVE) {
+                    double pv[ROWS_PER_WAVE], sv[ROWS_PER_WAVE];
+#pragma unroll
+                    for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                        pv[u] = M.P[(size_t)ii[u] * n + j];
+                        sv[u] = M.S[(size_t)ii[u] * n + j];
+                    }
+#pragma unroll
+                    for (int u = 0; u < ROWS_PER_WAVE; ++u) sq[u] = fma(pv[u], sv[u], sq[u]);
                 }
             }
-            sq = wave_sum(sq);
-            se = wave_sum(se);
-            if (l == 0) {
-                M.q[i] = (i < M.nao) ? b.n0[M.vec0 + i] - sq : 0.0;
-                M.eorb[i] = (i < M.nao) ? se : 0.0;
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                const double a = wave_sum(sq[u]), e = wave_sum(se[u]);
+                const int i = i0 + u;
+                if (l == 0 && i < n) {
+                    M.q[i] = (i < M.nao) ? b.n0[M.vec0 + i] - a : 0.0;
+                    M.eorb[i] = (i < M.nao) ? e : 0.0;
+                }
             }
         }
     }

@@ -90,6 +90,24 @@ __device__ inline void wg_sum_n(double (&vals)[K], ldsf* red) {
     __syncthreads();
 }
 
+// Y[e] = a X[e] + b Z[e] (Z null: a X[e]) over `len` contiguous elements, by
+// the workgroup, four elements per thread in flight.  Y may alias X or Z
+// (element-wise).  No barrier inside.
+__device__ inline void wg_axpby(size_t len, double a, const glbf* X, double b, const glbf* Z, glbf* Y) {
+    size_t e = threadIdx.x;
+    for (; e + 3 * NT < len; e += 4 * NT) {
+        double x[4], z[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            x[u] = X[e + u * NT];
+            z[u] = Z ? Z[e + u * NT] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) Y[e + u * NT] = Z ? a * x[u] + b * z[u] : a * x[u];
+    }
+    for (; e < len; e += NT) Y[e] = Z ? a * X[e] + b * Z[e] : a * X[e];
+}
+
 // ---------------------------------------------------------------------------
 // FP64 matrix cores.  v_mfma_f64_16x16x4_f64: D(16x16) += A(16x4) B(4x16);
 // lane l supplies A[l%16][l/16] and B[l/16][l%16] and holds
