@@ -398,6 +398,9 @@ def main() -> int:
     ap.add_argument("--profile-serving", action="store_true",
                     help="only serve 8 chunks through the continuous-batching loop (kernel traces)")
     ap.add_argument("--no-large", action="store_true", help="skip the 600-water single system")
+    ap.add_argument("--no-latency", action="store_true", help="skip the latency runs")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the forces-cost, MD and device-peak measurements")
     ap.add_argument("--profile-large", action="store_true",
                     help="only the 600-water single system, energy + forces once (kernel traces)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -546,7 +549,7 @@ def main() -> int:
     gc.collect()
     gc.freeze()
     curve = []
-    for frac in [float(x) for x in a.latency_rates.split(",")]:
+    for frac in ([] if a.no_latency else [float(x) for x in a.latency_rates.split(",")]):
         r_ = frac * value
         lat_reqs = _requests(stream, 0, a.latency_requests)
         rng = np.random.default_rng(44)
@@ -564,7 +567,8 @@ def main() -> int:
         ls.results.clear()
         del lat_reqs, ls
         gc.collect()
-    head = min(curve, key=lambda c: abs(c["rate_frac"] - 0.7))
+    head = min(curve, key=lambda c: abs(c["rate_frac"] - 0.7)) if curve else {
+        "rate_per_s": None, "p50_ms": None, "p99_ms": None}
 
     padded = {}
     if not a.no_padded:
@@ -574,13 +578,13 @@ def main() -> int:
             log(f"padded drop-in, batches of {pb}: {padded[pb]:.1f} molecules/s")
     pb0 = int(a.padded_batch.split(",")[0])
 
-    fc = forces_cost(_requests(stream, a.warmup * a.chunk, min(a.chunk, 128)), dev, opts)
+    fc = None if a.no_extras else forces_cost(_requests(stream, a.warmup * a.chunk, min(a.chunk, 128)), dev, opts)
     log(f"forces cost: {fc}")
-    md = md_warm_start(dev, opts)
+    md = None if a.no_extras else md_warm_start(dev, opts)
     log(f"md warm start: {md}")
     big = None if a.no_large else large_system(dev, opts)
     log(f"600-water cluster: {big}")
-    peaks = device_peaks(dev)
+    peaks = None if a.no_extras else device_peaks(dev)
     log(f"measured peaks: {peaks}")
 
     cpu = None if a.no_cpu else cpu_baseline(stream, a.cpu_seconds)

@@ -787,16 +787,33 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     // ES2 (gamma q_sh) + ES3 (hd q_at^2) shell potential, one wave per row
     {
         const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
-        for (int s = w; s < M.nsh; s += NWAVE) {
-            const glbf* gr = M.gam + (size_t)s * M.nsh;
-            double acc = 0.0;
-            for (int t = l; t < M.nsh; t += WAVE) acc = fma(gr[t], sh.qsh[t], acc);
-            acc = wave_sum(acc);
-            if (l == 0) {
-                const int a = b.sh_atom[M.sh0 + s];
-                const double hd = par.sp_gam3[b.atom_sp[M.at0 + a]];
-                sh.gq[s] = acc;
-                sh.vsh[s] = acc + hd * sh.qat[a] * sh.qat[a];
+        const int m = M.nsh;
+        for (int s0 = w * ROWS_PER_WAVE; s0 < m; s0 += NWAVE * ROWS_PER_WAVE) {
+            int ss[ROWS_PER_WAVE];
+            double acc[ROWS_PER_WAVE];
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                ss[u] = s0 + u < m ? s0 + u : m - 1;
+                acc[u] = 0.0;
+            }
+            for (int t = l; t < m; t += WAVE) {
+                const double qt = sh.qsh[t];
+                double g[ROWS_PER_WAVE];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) g[u] = M.gam[(size_t)ss[u] * m + t];
+#pragma unroll
+                for (int u = 0; u < ROWS_PER_WAVE; ++u) acc[u] = fma(g[u], qt, acc[u]);
+            }
+#pragma unroll
+            for (int u = 0; u < ROWS_PER_WAVE; ++u) {
+                const double v = wave_sum(acc[u]);
+                const int s = s0 + u;
+                if (l == 0 && s < m) {
+                    const int a = b.sh_atom[M.sh0 + s];
+                    const double hd = par.sp_gam3[b.atom_sp[M.at0 + a]];
+                    sh.gq[s] = v;
+                    sh.vsh[s] = v + hd * sh.qat[a] * sh.qat[a];
+                }
             }
         }
     }

@@ -922,11 +922,24 @@ __device__ __noinline__ int wg_block_jacobi(glbf* A, int n, CAcc Ct, glbf* Qbuf,
     __syncthreads();
     int sweep = 0;
     for (;;) {
+        // (four rows per wave: the loads of four rows in flight)
         double off = 0.0;
-        for (int i = w; i < n; i += NWAVE) {
-            const int ci = occ_class(i, cb_lo, cb_hi);
-            for (int j = i + 1 + lane; j < n; j += WAVE)
-                if (pair_in_mode(mode, ci, occ_class(j, cb_lo, cb_hi))) off = fmax(off, fabs(A[(size_t)i * n + j]));
+        for (int i0 = w * 4; i0 < n; i0 += NWAVE * 4) {
+            int ii[4], ci[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                ii[u] = i0 + u < n ? i0 + u : n - 1;
+                ci[u] = occ_class(ii[u], cb_lo, cb_hi);
+            }
+            for (int j = i0 + 1 + lane; j < n; j += WAVE) {
+                double a[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) a[u] = A[(size_t)ii[u] * n + j];
+                const int cj = occ_class(j, cb_lo, cb_hi);
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (j > i0 + u && i0 + u < n && pair_in_mode(mode, ci[u], cj)) off = fmax(off, fabs(a[u]));
+            }
         }
         off = wg_max(off, red);
         if (off <= tol) return sweep;
