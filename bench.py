@@ -373,7 +373,7 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--lookahead", type=int, default=4,
+    ap.add_argument("--lookahead", type=int, default=6,
                     help="chunks the serving loop holds (waiting or in flight) at most")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-16)")

@@ -32,7 +32,71 @@ __global__ void __launch_bounds__(1024) k_zero2(double* __restrict__ a, int64_t 
 
 #define XTB_MAX_US_LDS_F 64
 
-__global__ void __launch_bounds__(256) k_grad_pairs(xtb_batch b, xtb_params par,
+// One shell pair's part of the gradient, contracted inside the primitive
+// loop (no dS block is formed): with w_xy = 2 (P_xy ht - W_xy) - P_xy
+// (v_x + v_y) over the pair's block (rows of shell i on A, columns of shell
+// j on B, leading dimension n), gvec += sum_xy w_xy d<x|y>/dR_A and
+// phs += sum_xy P_xy <x|y>.  The primitive-pair terms are those of
+// shell_overlap_t (McMurchie-Davidson, p order y, z, x).
+template <int LI, int LJ>
+__device__ inline void grad_shell_pair(int npi, const double* ai, const double* ci, int npj,
+                                       const double* aj, const double* cj, const double R[3],
+                                       const double* __restrict__ Pb, const double* __restrict__ Wb,
+                                       int n, const double* vi, const double* vj, double ht,
+                                       double gvec[3], double& phs) {
+    constexpr int NI = 2 * LI + 1, NJ = 2 * LJ + 1;
+    double pm[NI][NJ], wv[NI][NJ];
+#pragma unroll
+    for (int x = 0; x < NI; ++x)
+#pragma unroll
+        for (int y = 0; y < NJ; ++y) {
+            const double pmn = Pb[(size_t)x * n + y];
+            pm[x][y] = pmn;
+            wv[x][y] = 2.0 * (pmn * ht - Wb[(size_t)x * n + y]) - pmn * (vi[x] + vj[y]);
+        }
+    const double r2 = R[0] * R[0] + R[1] * R[1] + R[2] * R[2];
+    for (int u = 0; u < npi; ++u) {
+        const double a = ai[u];
+        for (int v = 0; v < npj; ++v) {
+            const double b = aj[v];
+            const double p = a + b;
+            const double ip = 1.0 / p;
+            const double mu = a * b * ip;
+            if (mu * r2 > XTB_PRIM_SCREEN) continue;
+            const double s00 = ci[u] * cj[v] * (XTB_PI * ip) * sqrt(XTB_PI * ip) * exp(-mu * r2);
+            const double pa = -b * ip;  // (P - A) = pa R
+            const double pb = a * ip;   // (P - B) = pb R
+#pragma unroll
+            for (int mi = 0; mi < NI; ++mi) {
+                const int x = LI ? (mi + 1) % 3 : -1;
+#pragma unroll
+                for (int mj = 0; mj < NJ; ++mj) {
+                    const int y = LJ ? (mj + 1) % 3 : -1;
+                    double f;
+                    if (x < 0 && y < 0) f = 1.0;
+                    else if (y < 0) f = pa * R[x];
+                    else if (x < 0) f = pb * R[y];
+                    else f = pa * pb * R[x] * R[y] + (x == y ? 0.5 * ip : 0.0);
+                    phs = fma(pm[mi][mj], f * s00, phs);
+                    const double wf = wv[mi][mj] * s00;
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        double df = 0.0;
+                        if (x >= 0 && y < 0) df = (k == x) ? pa : 0.0;
+                        else if (x < 0 && y >= 0) df = (k == y) ? pb : 0.0;
+                        else if (x >= 0 && y >= 0) {
+                            if (k == x) df += pa * pb * R[y];
+                            if (k == y) df += pa * pb * R[x];
+                        }
+                        gvec[k] = fma(wf, df - 2.0 * mu * R[k] * f, gvec[k]);
+                    }
+                }
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256, 3) k_grad_pairs(xtb_batch b, xtb_params par,
                                                     const double* __restrict__ pos,
                                                     const double* __restrict__ cn,
                                                     const double* __restrict__ gam,
@@ -97,29 +161,22 @@ __global__ void __launch_bounds__(256) k_grad_pairs(xtb_batch b, xtb_params par,
             radial -= gij * gij * gij * qsh[i] * qsh[j];
             if (!in_cut) continue;
             const double hj = par.us_level[usj] - par.us_kcn[usj] * cn[gb];
-            double Sb[3][3], dSb[3][3][3];
-            shell_overlap<true>(li, lj, par.us_nprim[usi], s_alpha + usi * XTB_MAXPRIM,
-                                s_coeff + usi * XTB_MAXPRIM, par.us_nprim[usj],
-                                s_alpha + usj * XTB_MAXPRIM, s_coeff + usj * XTB_MAXPRIM, R, Sb,
-                                dSb);
             double pi, dpi;
             dist_poly(dist, rab, par.us_shpoly[usi], par.us_shpoly[usj], pi, dpi);
             const double kk = par.us_K[usi * par.nus + usj];
             const double ht = 0.5 * (hi + hj) * kk * pi;
-            double phs = 0.0, ps = 0.0;
-            for (int x = 0; x < ni; ++x) {
-                const double vm = v[oi + x];
-                for (int y = 0; y < nj; ++y) {
-                    const size_t e = (size_t)(oi + x) * n + (oj + y);
-                    const double pmn = P[e], wmn = W[e];
-                    const double sval = 2.0 * (pmn * ht - wmn) - pmn * (vm + v[oj + y]);
-                    gvec[0] += sval * dSb[0][x][y];
-                    gvec[1] += sval * dSb[1][x][y];
-                    gvec[2] += sval * dSb[2][x][y];
-                    phs += pmn * Sb[x][y];
-                    ps += pmn * Sb[x][y];
-                }
-            }
+            double phs = 0.0;
+            (void)ni;
+            (void)nj;
+            dispatch_l(li, lj, [&](auto LIc, auto LJc) {
+                constexpr int LI = decltype(LIc)::value, LJ = decltype(LJc)::value;
+                grad_shell_pair<LI, LJ>(par.us_nprim[usi], s_alpha + usi * XTB_MAXPRIM,
+                                        s_coeff + usi * XTB_MAXPRIM, par.us_nprim[usj],
+                                        s_alpha + usj * XTB_MAXPRIM, s_coeff + usj * XTB_MAXPRIM, R,
+                                        P + (size_t)oi * n + oj, W + (size_t)oi * n + oj, n, v + oi, v + oj,
+                                        ht, gvec, phs);
+            });
+            const double ps = phs;
             radial += 2.0 * phs * ht * dpi / pi;
             const double fac = kk * pi;
             dA -= ps * fac * par.us_kcn[usi];

@@ -113,40 +113,55 @@ __device__ inline void shell_pair_s_h0(const xtb_batch& b, const xtb_params& par
     const int oi = b.sh_ao[sh0 + i], oj = b.sh_ao[sh0 + j];
     const double hi = par.us_level[usi] - par.us_kcn[usi] * cni;
     const double hj = par.us_level[usj] - par.us_kcn[usj] * cnj;
-    double blk[3][3], hblk[3][3];
-    for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y) { blk[x][y] = 0.0; hblk[x][y] = 0.0; }
-    if (ati == atj) {
-        if (i == j)
-            for (int x = 0; x < ni; ++x) { blk[x][x] = 1.0; hblk[x][x] = hi; }
-    } else {
-        const int ga = at0 + ati, gb = at0 + atj;
-        double R[3] = {pos[3 * ga] - pos[3 * gb], pos[3 * ga + 1] - pos[3 * gb + 1],
-                       pos[3 * ga + 2] - pos[3 * gb + 2]};
-        const double dist = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
-        if (dist < par.int_cutoff && dist > 0.1) {
-            double dummy[3][3][3];
-            shell_overlap<false>(li, lj, par.us_nprim[usi], s_alpha + usi * XTB_MAXPRIM,
-                                 s_coeff + usi * XTB_MAXPRIM, par.us_nprim[usj],
-                                 s_alpha + usj * XTB_MAXPRIM, s_coeff + usj * XTB_MAXPRIM, R,
-                                 blk, dummy);
-            const double rab = par.sp_rad[b.atom_sp[ga]] + par.sp_rad[b.atom_sp[gb]];
-            double pi, dpi;
-            dist_poly(dist, rab, par.us_shpoly[usi], par.us_shpoly[usj], pi, dpi);
-            const double fac = 0.5 * (hi + hj) * par.us_K[usi * par.nus + usj] * pi;
-            for (int x = 0; x < ni; ++x)
-                for (int y = 0; y < nj; ++y) hblk[x][y] = blk[x][y] * fac;
+    (void)ni;
+    (void)nj;
+    dispatch_l(li, lj, [&](auto LIc, auto LJc) {
+        constexpr int LI = decltype(LIc)::value, LJ = decltype(LJc)::value;
+        constexpr int NI = 2 * LI + 1, NJ = 2 * LJ + 1;
+        double blk[NI][NJ], hblk[NI][NJ];
+#pragma unroll
+        for (int x = 0; x < NI; ++x)
+#pragma unroll
+            for (int y = 0; y < NJ; ++y) { blk[x][y] = 0.0; hblk[x][y] = 0.0; }
+        if (ati == atj) {
+            if constexpr (LI == LJ) {
+                if (i == j)
+#pragma unroll
+                    for (int x = 0; x < NI; ++x) { blk[x][x] = 1.0; hblk[x][x] = hi; }
+            }
+        } else {
+            const int ga = at0 + ati, gb = at0 + atj;
+            double R[3] = {pos[3 * ga] - pos[3 * gb], pos[3 * ga + 1] - pos[3 * gb + 1],
+                           pos[3 * ga + 2] - pos[3 * gb + 2]};
+            const double dist = sqrt(R[0] * R[0] + R[1] * R[1] + R[2] * R[2]);
+            if (dist < par.int_cutoff && dist > 0.1) {
+                double dummy[3][NI][NJ];
+                shell_overlap_t<false, LI, LJ>(par.us_nprim[usi], s_alpha + usi * XTB_MAXPRIM,
+                                               s_coeff + usi * XTB_MAXPRIM, par.us_nprim[usj],
+                                               s_alpha + usj * XTB_MAXPRIM, s_coeff + usj * XTB_MAXPRIM, R,
+                                               blk, dummy);
+                const double rab = par.sp_rad[b.atom_sp[ga]] + par.sp_rad[b.atom_sp[gb]];
+                double pi, dpi;
+                dist_poly(dist, rab, par.us_shpoly[usi], par.us_shpoly[usj], pi, dpi);
+                const double fac = 0.5 * (hi + hj) * par.us_K[usi * par.nus + usj] * pi;
+#pragma unroll
+                for (int x = 0; x < NI; ++x)
+#pragma unroll
+                    for (int y = 0; y < NJ; ++y) hblk[x][y] = blk[x][y] * fac;
+            }
         }
-    }
-    for (int x = 0; x < ni; ++x)
-        for (int y = 0; y < nj; ++y) {
-            const size_t e1 = mat0 + (size_t)(oi + x) * n + (oj + y);
-            const size_t e2 = mat0 + (size_t)(oj + y) * n + (oi + x);
-            S[e1] = blk[x][y];
-            H0[e1] = hblk[x][y];
-            S[e2] = blk[x][y];
-            H0[e2] = hblk[x][y];
-        }
+#pragma unroll
+        for (int x = 0; x < NI; ++x)
+#pragma unroll
+            for (int y = 0; y < NJ; ++y) {
+                const size_t e1 = mat0 + (size_t)(oi + x) * n + (oj + y);
+                const size_t e2 = mat0 + (size_t)(oj + y) * n + (oi + x);
+                S[e1] = blk[x][y];
+                H0[e1] = hblk[x][y];
+                S[e2] = blk[x][y];
+                H0[e2] = hblk[x][y];
+            }
+    });
 }
 
 __global__ void __launch_bounds__(256) k_s_h0(xtb_batch b, xtb_params par,
