@@ -35,7 +35,7 @@ import torch
 
 from . import _lib
 from . import constants as C
-from .engine import BatchLayout, DeviceTables, Pipeline, ScfOptions
+from .engine import WG_MAX_N, BatchLayout, DeviceTables, Pipeline, ScfOptions, SplitPipeline
 from .exceptions import (
     DeviceError,
     DtypeError,
@@ -47,7 +47,9 @@ from .param import SpeciesTables
 
 _DISP_WARNED = False
 # a single molecule with more orbitals than this runs its SCF on the whole
-# GPU (xtb_scf_large); batches use the per-molecule kernel (limit 1024)
+# GPU (xtb_scf_large); batches use the per-molecule kernel, except for their
+# molecules above its limit (WG_MAX_N orbitals), which take the whole GPU
+# one after the other
 LARGE_N = 600
 
 
@@ -224,9 +226,13 @@ class GFN1Calculator:
                 self._dtables = DeviceTables(self.tables, self.device)
             layout = BatchLayout(self._mols, self.tables, padded=self.batched and self.layout_padded,
                                  chrg=c, spin=sp)
-            # one molecule above LARGE_N orbitals: its SCF on the whole GPU
+            # one molecule above LARGE_N orbitals: its SCF on the whole GPU;
+            # in a batch, the molecules above the per-molecule kernel's limit
             large = layout.B == 1 and not layout.padded and int(layout.n[0]) > LARGE_N
-            self._pipe = Pipeline(layout, self._dtables, self.device, large=large)
+            if not large and np.any(layout.nao > WG_MAX_N):
+                self._pipe = SplitPipeline(layout, self._dtables, self.device)
+            else:
+                self._pipe = Pipeline(layout, self._dtables, self.device, large=large)
             self._pipe_key = key
         return self._pipe
 

@@ -599,3 +599,109 @@ class Pipeline:
         n = int(self.L.n[b])
         m0 = int(self.L.mat0[b])
         return buf[m0: m0 + n * n].view(n, n)
+
+
+# the per-molecule SCF kernel's size limit (orbitals of one molecule)
+WG_MAX_N = 1024
+
+
+class SplitPipeline:
+    """A batch holding molecules above the per-molecule kernel's limit
+    (WG_MAX_N orbitals): each of those runs its SCF on the whole GPU
+    (xtb_scf_large), one after the other, and the remaining molecules run
+    together as a batch of their own (xtb_scf), on the same stream.  The
+    accessors return the results in the order and shapes of the original
+    layout.  Upstream runs any batch through one padded SCF
+    (src/dxtb/_src/scf/unrolling/base.py:122-139); in a padded batch the
+    molecules here are padded to the largest of their own part, which drops
+    only the Fermi occupations of the zero-energy padding levels that
+    upstream's larger padding adds, exp(eF / kT) of a level each."""
+
+    large = False
+
+    def __init__(self, layout: BatchLayout, dtables: DeviceTables, device):
+        self.L = L = layout
+        self.device = torch.device(device)
+        big = np.nonzero(L.nao > WG_MAX_N)[0]
+        small = np.nonzero(L.nao <= WG_MAX_N)[0]
+        groups = ([(small, False)] if len(small) else []) + [(np.array([b]), True) for b in big]
+        spin = L.spin
+        self.parts = []
+        for idx, large in groups:
+            sp = None if spin is None else np.broadcast_to(np.asarray(spin, dtype=np.float64), (L.B,))[idx]
+            sub = BatchLayout([L.numbers[b] for b in idx], L.tables, padded=L.padded and not large,
+                              chrg=np.asarray(L.chrg, dtype=np.float64)[idx], spin=sp)
+            atoms = np.concatenate([np.arange(L.atom0[b], L.atom0[b + 1]) for b in idx])
+            self.parts.append((torch.as_tensor(idx, device=self.device),
+                               torch.as_tensor(atoms, device=self.device),
+                               Pipeline(sub, dtables, device, large=large)))
+        # molecule -> (part, index in the part)
+        self._where = {}
+        for k, (idx, _, _) in enumerate(self.parts):
+            for j, b in enumerate(idx.tolist()):
+                self._where[b] = (k, j)
+        self.out = torch.zeros((L.B, _lib.OUT_STRIDE), dtype=torch.float64, device=self.device)
+        self.grad = torch.zeros((int(L.nat.sum()), 3), dtype=torch.float64, device=self.device)
+
+    def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None, **kw) -> None:
+        if positions.shape != (int(self.L.nat.sum()), 3):
+            raise ValueError("positions do not match the batch layout")
+        st = stream if stream is not None else torch.cuda.current_stream(self.device)
+        with torch.cuda.stream(st):
+            for idx, atoms, p in self.parts:
+                p.run(positions.index_select(0, atoms).contiguous(), opts, forces=forces, stream=st)
+                self.out.index_copy_(0, idx, p.records())
+                if forces:
+                    self.grad.index_copy_(0, atoms, p.gradient())
+
+    def records(self) -> torch.Tensor:
+        return self.out
+
+    def energies(self) -> torch.Tensor:
+        r = self.out
+        return r[:, _lib.OUT_ESCF] + r[:, _lib.OUT_FENERGY] + r[:, _lib.OUT_EREP]
+
+    def gradient(self) -> torch.Tensor:
+        return self.grad
+
+    def repulsion_gradient(self, stream=None) -> torch.Tensor:
+        g = torch.zeros_like(self.grad)
+        for _, atoms, p in self.parts:
+            g.index_copy_(0, atoms, p.repulsion_gradient(stream))
+        return g
+
+    def orbital_charges(self, b: int) -> torch.Tensor:
+        k, j = self._where[b]
+        return self.parts[k][2].orbital_charges(j)
+
+    @property
+    def emo(self) -> torch.Tensor:
+        """Orbital energies in the original layout's vectors (L.vec0, L.n):
+        in a padded batch a molecule of a part padded less than the batch
+        gets the missing padding levels' energy, zero, sorted in."""
+        L = self.L
+        out = torch.zeros(L.nvec_total, dtype=torch.float64, device=self.device)
+        for b in range(L.B):
+            k, j = self._where[b]
+            p = self.parts[k][2]
+            v0, n = int(p.L.vec0[j]), int(p.L.n[j])
+            e = p.emo[v0:v0 + n]
+            m = int(L.n[b])
+            if m > n:
+                e = torch.sort(torch.cat([e, e.new_zeros(m - n)])).values
+            out[int(L.vec0[b]):int(L.vec0[b]) + m] = e
+        return out
+
+    def param_grad(self, stream=None) -> torch.Tensor:
+        rows = [(idx, p.param_grad(stream)) for idx, _, p in self.parts]
+        g = torch.zeros((self.L.B, rows[0][1].shape[1]), dtype=torch.float64, device=self.device)
+        for idx, r in rows:
+            g.index_copy_(0, idx, r)
+        return g
+
+    def param_grad_basis(self, tables, rel_step: float = 1e-4, stream=None) -> torch.Tensor:
+        rows = [(idx, p.param_grad_basis(tables, rel_step, stream)) for idx, _, p in self.parts]
+        g = torch.zeros((self.L.B, rows[0][1].shape[1]), dtype=torch.float64, device=self.device)
+        for idx, r in rows:
+            g.index_copy_(0, idx, r)
+        return g

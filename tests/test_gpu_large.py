@@ -125,3 +125,53 @@ def test_config3_water600_vs_oracle_fixture():
     qat = np.zeros(1800)
     np.add.at(qat, ao_atom, q.cpu().numpy())
     assert np.abs(qat - ref["charges_atom"]).max() < 1e-7
+
+
+@pytest.mark.parametrize("padded", [False, True])
+def test_batch_with_molecule_above_kernel_limit(fx, padded):
+    """A batch through the calculator whose second molecule (129 H2O, 1032
+    orbitals) exceeds the per-molecule kernel's limit (1024): it runs on the
+    whole-GPU path and the other two as a batch of their own.  Energies,
+    orbital charges, forces and orbital energies of all three, in the
+    batch's order, against the oracle.  Upstream pads the small molecules to
+    1032 orbitals; the oracle with that padding (npad) is the reference, and
+    at 300 K its padding levels take no occupation to double precision."""
+    from dxtb_amd import GFN1Calculator
+    from dxtb_amd.engine import SplitPipeline
+
+    rng = np.random.default_rng(41)
+    mols = [mol(fx, "h2o"), W.water_cluster(rng, 129), W.chain(rng, 8)]
+    nums, pos = W.pad(mols)
+    dev = torch.device("cuda:0")
+    print(f"case: batch with a 1032-orbital molecule, padded={padded}", flush=True)
+    calc = GFN1Calculator(torch.tensor(nums, device=dev), dtype=torch.float64, device=dev,
+                          opts={**TIGHT_ORACLE, "exclude": ["disp"]}, padded=padded)
+    p = torch.tensor(pos, dtype=torch.float64, device=dev)
+    e = calc.get_energy(p).cpu().numpy()
+    assert isinstance(calc._pipe, SplitPipeline)
+    f = calc.forces_analytical(p).cpu().numpy()
+    q = calc.get_charges(p).cpu().numpy()
+    emo = calc.get_mo_energies(p)
+    nmax = 1032
+    for b, (z, x) in enumerate(mols):
+        o = og.singlepoint(z, x, opts=TIGHT_ORACLE, with_forces=True)
+        nao = o["charges_orbital"].shape[0]
+        if b == 1:
+            assert nao == nmax
+        assert abs(e[b] - o["energy"]) < 1e-9, (b, e[b], o["energy"])
+        assert np.abs(q[b, :nao] - o["charges_orbital"].numpy()).max() < 1e-8
+        assert np.all(q[b, nao:] == 0.0)
+        assert np.abs(-f[b, : len(z)] - o["gradient"].numpy()).max() < 1e-8
+        assert np.all(f[b, len(z):] == 0.0)
+        eb = emo[b].cpu().numpy()
+        if padded:
+            # upstream's padded spectrum: the molecule's levels and
+            # nmax - nao padding levels at zero, sorted
+            assert eb.shape == (nmax,)
+            ref = np.sort(np.concatenate([o["emo"].numpy(), np.zeros(nmax - nao)]))
+            if b != 1:
+                op = og.singlepoint(z, x, opts=TIGHT_ORACLE, npad=nmax - nao)
+                assert abs(op["energy"] - o["energy"]) < 1e-12
+        else:
+            ref = o["emo"].numpy()
+        assert np.abs(eb - ref).max() < 1e-8
