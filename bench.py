@@ -16,12 +16,14 @@ carries
 * p50/p99 latency of the serving loop on the whole 20000-request stream under
   Poisson arrivals at 30/50/70/90 % of the measured saturation rate
   (`latency_curve`; the headline p50/p99 at 70 %);
-* a roofline entry for the dominant kernel (the persistent SCF kernel,
-  FP64): algorithmic flops / duration from HIP events of one launch over
-  requests 600-2999 (as many as the serving loop holds), alone on the GPU,
-  with its HBM traffic and executed matrix-core flops from the committed
-  rocprofv3 summaries of that launch, and the stage rates of the integral
-  pass and of the kernel's Fock-build and Mulliken stages;
+* a roofline entry for the dominant kernel (the fused single-point kernel
+  the serving loop launches: each molecule's integral pass, SCF and forces
+  on its own CU, FP64): the SCF's algorithmic flops / duration from HIP
+  events of one launch over requests 600-2999 (four chunks), alone on the
+  GPU, with its HBM traffic and executed matrix-core flops from the
+  committed rocprofv3 summaries of that launch; the SCF kernel alone on the
+  same batch beside it, and the stage rates of the integral pass and of the
+  SCF's Fock-build and Mulliken stages;
 * the CPU baseline: the oracle (a PyTorch FP64 restatement of the upstream
   algorithm) timed on this host on a bounded prefix of the same stream.
 
@@ -276,19 +278,40 @@ def roofline_pass(stream, device, opts, start: int = ROOFLINE_START, count: int 
     # one of the serving loop's process streams: every stream beyond the
     # hardware queues shares a queue, and later serving runs slowed down
     st = _process_streams(torch.device(device), 1)[0]
+    dt = DeviceTables(tb, device)
     with torch.cuda.stream(st):
-        pipe = Pipeline(lay, DeviceTables(tb, device), device)
+        pipe = Pipeline(lay, dt, device)
         pos = torch.as_tensor(np.concatenate([r.positions for r in reqs]), dtype=torch.float64).to(device)
         pipe.run(pos, opts, forces=True, stream=st, timing=True)
+        ef = torch.cuda.Event(enable_timing=True)
+        ef.record(st)
     st.synchronize()
     ev0, ev1 = pipe.scf_events
     ms = float(ev0.elapsed_time(ev1))
+    forces_ms = float(ev1.elapsed_time(ef))
     ei0, ei1 = pipe.int_events
     int_ms = float(ei0.elapsed_time(ei1))
     fl = pipe.scf_flops()
     rec = pipe.records().cpu().numpy()
     bad = int((rec[:, _lib.OUT_STATUS] != 0).sum())
     achieved = fl / (ms * 1e-3) / 1e12
+    # the same batch as ONE fused launch (xtb_singlepoint: each molecule's
+    # integral pass, SCF and gradient on its own CU), as the serving loop runs it
+    with torch.cuda.stream(st):
+        fpipe = Pipeline(lay, dt, device)
+        fpipe.run(pos, opts, forces=True, stream=st, timing=True, fused=True)
+    st.synchronize()
+    fv0, fv1 = fpipe.scf_events
+    fms = float(fv0.elapsed_time(fv1))
+    frec = fpipe.records().cpu().numpy()
+    fbusy = float(frec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
+    fused = {"kernel": "xtb::k_singlepoint", "mean_launch_ms": fms, "achieved": fl / (fms * 1e-3) / 1e12,
+             "cu_occupancy": fbusy / (NCU * fms * 1e-3),
+             "longest_molecule_ms": float(frec[:, _lib.OUT_T_TOTAL].max()) * 1e-3,
+             "integrals_workgroup_s": float(frec[:, _lib.OUT_T_INTEGRALS].sum()) * 1e-6,
+             "forces_workgroup_s": float(frec[:, _lib.OUT_T_FORCES].sum()) * 1e-6,
+             "same_energies": bool(np.array_equal(pipe.energies().cpu().numpy(), fpipe.energies().cpu().numpy())),
+             "three_launches_ms": int_ms + ms + forces_ms}
     # the same flops over the workgroups' own device time (one workgroup per
     # molecule, one CU each): the kernel's efficiency while resident,
     # independent of how many CUs are busy
@@ -325,10 +348,11 @@ def roofline_pass(stream, device, opts, start: int = ROOFLINE_START, count: int 
                                "bytes_rule": "16 n^2 (P, S; +8 n^2 H0 in the last pass) + 8 m^2 (gamma)"},
         "cu_share_gbs": share,
     }
+    stages["forces_ms"] = round(forces_ms, 3)
     return {"achieved": achieved, "launches": 1, "mean_launch_ms": ms, "flops_per_launch": fl,
             "molecules": count, "requests": f"{start}-{start + count - 1}", "nonconverged": bad,
             "resident_frac": fl / (busy * per_cu_peak), "cu_occupancy": busy / (NCU * ms * 1e-3),
-            "longest_molecule_ms": longest, "stages": stages}
+            "longest_molecule_ms": longest, "stages": stages, "fused": fused}
 
 
 def _profile_json(name: str, start: int, count: int) -> dict | None:
@@ -345,20 +369,21 @@ def _profile_json(name: str, start: int, count: int) -> dict | None:
     return t
 
 
-def measured_traffic(start: int = ROOFLINE_START, count: int = ROOFLINE_COUNT) -> float | None:
-    """HBM bytes of the roofline pass's k_scf launch, from the rocprofv3
-    FETCH_SIZE / WRITE_SIZE runs (copy-calibrated) summarised in
-    profiles/k_scf_traffic.json."""
-    t = _profile_json("k_scf_traffic.json", start, count)
+def measured_traffic(kernel: str, start: int = ROOFLINE_START, count: int = ROOFLINE_COUNT) -> float | None:
+    """HBM bytes of the roofline pass's launch of `kernel` (k_singlepoint or
+    k_scf), from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs (copy-calibrated)
+    summarised in profiles/<kernel>_traffic.json."""
+    t = _profile_json(f"{kernel}_traffic.json", start, count)
     return None if t is None else float(t["traffic_bytes_per_launch"])
 
 
-def measured_mfma(ms: float, start: int = ROOFLINE_START, count: int = ROOFLINE_COUNT) -> dict | None:
+def measured_mfma(kernel: str, ms: float, start: int = ROOFLINE_START,
+                  count: int = ROOFLINE_COUNT) -> dict | None:
     """Executed FP64 matrix-core work of the same launch from the rocprofv3
     SQ_INSTS_VALU_MFMA_MOPS_F64 run (units of 512 flops) summarised in
-    profiles/k_scf_mfma.json, over this run's launch time: what the matrix
+    profiles/<kernel>_mfma.json, over this run's launch time: what the matrix
     cores executed, next to the fixed-convention flops of `achieved`."""
-    t = _profile_json("k_scf_mfma.json", start, count)
+    t = _profile_json(f"{kernel}_mfma.json", start, count)
     if t is None:
         return None
     fl = float(t["mfma_f64_flops_per_launch"])
@@ -373,11 +398,19 @@ def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=40)
-    ap.add_argument("--lookahead", type=int, default=6,
+    ap.add_argument("--lookahead", type=int, default=8,
                     help="chunks the serving loop holds (waiting or in flight) at most")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="HIP hardware queues of the process (sets GPU_MAX_HW_QUEUES, 1-16)")
     ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--merge-above", type=int, default=1200,
+                    help="waiting requests above which a launch takes --merge-cap of them whatever "
+                         "their size class, largest first (-1: class launches only)")
+    ap.add_argument("--merge-cap", type=int, default=2400)
+    ap.add_argument("--merge-oldest", action="store_true",
+                    help="merged launches take the oldest waiting requests instead of the largest")
+    ap.add_argument("--no-fused", action="store_true",
+                    help="serving launches as integrals, SCF and forces kernels instead of one fused kernel")
     ap.add_argument("--chunk", type=int, default=600)
     ap.add_argument("--latency-requests", type=int, default=20000,
                     help="requests of the stream served under Poisson arrivals per rate")
@@ -440,6 +473,8 @@ def main() -> int:
                           "bytes_written": y.numel() * 8}))
         return 0
     opts = ScfOptions(x_atol=1e-10, x_atol_max=1e-10, maxiter=300)
+    merge = {"merge_above": a.merge_above if a.merge_above >= 0 else None, "merge_cap": a.merge_cap,
+             "merge_largest": not a.merge_oldest, "fused": not a.no_fused}
     if a.profile_large:
         print(json.dumps({"profile_large": large_system(dev, opts, repeats=1)}))
         return 0
@@ -453,7 +488,7 @@ def main() -> int:
         rp = roofline_pass(stream, dev, opts)
         print(json.dumps({"profile_run": rp}))
         return 0
-    server = RaggedServer(dev, opts=opts)
+    server = RaggedServer(dev, opts=opts, **merge)
     if a.profile_serving:
         # the continuous-batching loop over 8 chunks, after one warm-up chunk
         server.run_stream(_requests(stream, 0, a.chunk), realtime=False)
@@ -522,7 +557,7 @@ def main() -> int:
     # the same chunks one at a time, each drained before the next starts
     barrier = None
     if not a.no_barrier:
-        bsrv = RaggedServer(dev, opts=opts)
+        bsrv = RaggedServer(dev, opts=opts, **merge)
         bsrv.run_stream(_requests(stream, 0, a.chunk), realtime=False)  # warm-up
         torch.cuda.synchronize()
         tb0 = time.perf_counter()
@@ -535,10 +570,14 @@ def main() -> int:
 
     # roofline of the SCF kernel: one launch over the requests the loop holds
     rp = roofline_pass(stream, dev, opts)
-    achieved = rp["achieved"]
-    log(f"scf kernel: one launch of {rp['molecules']} molecules, {rp['mean_launch_ms']:.2f} ms, "
-        f"{achieved:.3f} TFLOP/s, CU occupancy {rp['cu_occupancy']:.3f}, "
-        f"longest molecule {rp['longest_molecule_ms']:.1f} ms")
+    fz = rp["fused"]
+    achieved = fz["achieved"]
+    log(f"fused kernel: one launch of {rp['molecules']} molecules, {fz['mean_launch_ms']:.2f} ms, "
+        f"{achieved:.3f} TFLOP/s, CU occupancy {fz['cu_occupancy']:.3f}, "
+        f"longest molecule {fz['longest_molecule_ms']:.1f} ms (integrals + SCF + forces: three launches "
+        f"{fz['three_launches_ms']:.2f} ms; same energies: {fz['same_energies']})")
+    log(f"scf kernel alone: {rp['mean_launch_ms']:.2f} ms, {rp['achieved']:.3f} TFLOP/s, "
+        f"CU occupancy {rp['cu_occupancy']:.3f}, longest molecule {rp['longest_molecule_ms']:.1f} ms")
     log(f"stages: {json.dumps(rp['stages'])}")
 
     # latency under Poisson arrivals at fractions of the saturation rate: the
@@ -557,7 +596,7 @@ def main() -> int:
         for r in lat_reqs:
             t += rng.exponential(1.0 / r_)
             r.arrival = t
-        ls = RaggedServer(dev, opts=opts)
+        ls = RaggedServer(dev, opts=opts, **merge)
         lat = ls.run_stream(lat_reqs, realtime=True)
         log(f"latency @ {r_:.0f}/s: p50 {lat['p50_ms']:.1f} ms p99 {lat['p99_ms']:.1f} ms "
             f"({lat['launches']} launches, host admission {ls.stats.host_admit_s:.3f} s, "
@@ -622,15 +661,29 @@ def main() -> int:
         },
         "roofline": {"bound": "mfma", "achieved": round(achieved, 4), "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": round(achieved / FP64_PEAK_TFLOPS, 5),
-                     "traffic": measured_traffic(),
-                     "kernel": "xtb::k_scf",
-                     "workload": f"one launch: requests {rp['requests']} of the stream, largest first",
-                     "launches": rp["launches"], "mean_launch_ms": round(rp["mean_launch_ms"], 3),
+                     "traffic": measured_traffic("k_singlepoint"),
+                     "kernel": "xtb::k_singlepoint",
+                     "workload": f"one launch: requests {rp['requests']} of the stream, largest first, "
+                                 "integral pass + SCF + forces per molecule (as the serving loop runs it)",
+                     "launches": 1, "mean_launch_ms": round(fz["mean_launch_ms"], 3),
                      "flops_per_launch": rp["flops_per_launch"],
-                     "resident_frac": round(rp["resident_frac"], 4),
-                     "cu_occupancy": round(rp["cu_occupancy"], 4),
-                     "longest_molecule_ms": round(rp["longest_molecule_ms"], 2),
-                     "matrix_cores": measured_mfma(rp["mean_launch_ms"]),
+                     "flops_rule": "the SCF's fixed convention (9 1/3 n^3 per step + density + "
+                                   "orthogonaliser + W); the integral pass and forces (FP64 VALU) add none",
+                     "cu_occupancy": round(fz["cu_occupancy"], 4),
+                     "longest_molecule_ms": round(fz["longest_molecule_ms"], 2),
+                     "matrix_cores": measured_mfma("k_singlepoint", fz["mean_launch_ms"]),
+                     "three_launches_ms": round(fz["three_launches_ms"], 3),
+                     "integrals_workgroup_s": round(fz["integrals_workgroup_s"], 4),
+                     "forces_workgroup_s": round(fz["forces_workgroup_s"], 4),
+                     "same_energies_as_three_launches": fz["same_energies"],
+                     "scf_kernel": {"kernel": "xtb::k_scf", "mean_launch_ms": round(rp["mean_launch_ms"], 3),
+                                    "achieved": round(rp["achieved"], 4),
+                                    "frac": round(rp["achieved"] / FP64_PEAK_TFLOPS, 5),
+                                    "traffic": measured_traffic("k_scf"),
+                                    "resident_frac": round(rp["resident_frac"], 4),
+                                    "cu_occupancy": round(rp["cu_occupancy"], 4),
+                                    "longest_molecule_ms": round(rp["longest_molecule_ms"], 2),
+                                    "matrix_cores": measured_mfma("k_scf", rp["mean_launch_ms"])},
                      "stages": rp["stages"],
                      "measured_peaks": peaks},
         "cpu_baseline": cpu,

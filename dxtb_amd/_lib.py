@@ -35,6 +35,7 @@ OUT_PD_STEPS, OUT_SWEEPS_FIRST, OUT_SWEEPS_LAST, OUT_T_PD = 17, 18, 19, 21
 OUT_T_START = 20  # device wall clock at the molecule's start, microseconds
 OUT_XCD = 22  # XCD the molecule's workgroup ran on
 OUT_T_FOCK, OUT_N_FOCK, OUT_T_MULL, OUT_N_MULL = 24, 25, 26, 27  # stage timers (us) and counts
+OUT_T_INTEGRALS, OUT_T_FORCES = 28, 29  # xtb_singlepoint: the molecule's integral pass and gradient (us)
 
 P = ct.c_void_p
 i32 = ct.c_int32
@@ -138,6 +139,9 @@ def load(path: str | os.PathLike | None = None):
     lib.xtb_scf_large_workspace.argtypes = [bp, ct.POINTER(i64)]
     lib.xtb_scf_large.restype = i32
     lib.xtb_scf_large.argtypes = [bp, pp, P, P, P, ct.POINTER(XtbScfOpts), ct.POINTER(XtbScfState), P, i64, P]
+    lib.xtb_singlepoint.restype = i32
+    lib.xtb_singlepoint.argtypes = [bp, pp, P, P, P, P, P, ct.POINTER(XtbScfOpts), ct.POINTER(XtbScfState),
+                                    P, P, P]
     if lib.xtb_abi_version() != 1:
         raise NativeLibraryError("libxtbhip.so ABI version mismatch")
     sizes = (i64 * 4)()
@@ -154,6 +158,7 @@ EXPORTED = (
     "xtb_scf", "xtb_scf_lds_bytes", "xtb_repulsion", "xtb_forces", "xtb_param_grad",
     "xtb_param_grad_basis", "xtb_build_s_h0_gamma", "xtb_repulsion_gradient",
     "xtb_forces_noalloc", "xtb_build_integrals", "xtb_scf_large_workspace", "xtb_scf_large",
+    "xtb_singlepoint",
 )
 
 

@@ -13,7 +13,7 @@ from pathlib import Path
 
 PKG = Path(__file__).resolve().parent
 ROOT = PKG.parent
-SOURCES = ["integrals.hip", "scf.hip", "large.hip", "forces.hip", "pgrad.hip"]
+SOURCES = ["integrals.hip", "scf.hip", "singlepoint.hip", "large.hip", "forces.hip", "pgrad.hip"]
 OUT = PKG / "libxtbhip.so"
 
 FLAGS = [

@@ -289,7 +289,9 @@ class HostArena:
     def need(layout: "BatchLayout") -> int:
         """Upper bound of the bytes one launch on `layout` takes."""
         nat = int(layout.nat.sum())
-        lay = sum(a.nbytes + 256 for a in layout.arrays.values()) + 4 * layout.B + 256
+        # the layout arrays, the launch order and the zeroed result records
+        lay = (sum(a.nbytes + 256 for a in layout.arrays.values()) + 4 * layout.B + 256
+               + 8 * layout.B * _lib.OUT_STRIDE + 256)
         return lay + 2 * (24 * nat + 256) + 8 * layout.B * _lib.OUT_STRIDE + 1024
 
     def reserve(self, nbytes: int) -> None:
@@ -334,6 +336,9 @@ class Pipeline:
         order = np.argsort(-layout.n, kind="stable").astype(np.int32)  # largest first
         arrays = dict(layout.arrays)
         arrays["order"] = order
+        # the result records start at zero (a warm start reads the previous
+        # status): zeros in the same upload instead of a fill kernel
+        arrays["out"] = np.zeros(layout.B * _lib.OUT_STRIDE)
         offs, total = {}, 0
         for k, a in arrays.items():
             a = np.ascontiguousarray(a)
@@ -366,7 +371,7 @@ class Pipeline:
         self.pot, self.vnew, self.q, self.eorb, self.emo = z(nv), z(nv), z(nv), z(nv), z(nv)
         self.occ = z(2 * nv)
         self.qsh = z(int(L.sh0[-1]))
-        self.out = torch.zeros(L.B * _lib.OUT_STRIDE, dtype=torch.float64, device=dev)
+        self.out = self.d["out"]
         self._kept = False  # the state buffers hold orbitals and potential to warm-start from
         self._pot_prev = None  # kept potential of the call before the last (extrapolation)
         self._have_w = False  # the last run formed the energy-weighted density
@@ -396,14 +401,17 @@ class Pipeline:
         )
 
     def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None,
-            timing: bool = False, wg_skip: int = 0) -> None:
+            timing: bool = False, wg_skip: int = 0, fused: bool = False) -> None:
         """positions: (nat_total, 3) float64 on the device, bohr.
 
         With ``timing`` HIP events bracket the SCF launch on its stream
         (``self.scf_events``) and the integral pass (CN, S, H0, gamma:
         ``self.int_events``, which includes the repulsion energy) for
         kernel-duration measurements.  ``wg_skip``
-        (0..7) shifts the SCF launch's workgroups to later XCDs."""
+        (0..7) shifts the SCF launch's workgroups to later XCDs.  ``fused``:
+        integrals, SCF and forces in one launch (xtb_singlepoint; the
+        gradient's per-atom sums in a fixed order); ``scf_events`` then
+        bracket that launch and there are no ``int_events``."""
         if positions.dtype != torch.float64 or not positions.is_contiguous():
             raise ValueError("positions must be contiguous float64")
         if positions.shape != (int(self.L.nat.sum()), 3):
@@ -417,15 +425,18 @@ class Pipeline:
             self.hist_x = torch.empty(g * self.L.nvec_total, dtype=torch.float64, device=self.device)
             self.hist_f = torch.empty_like(self.hist_x)
         pos = positions.data_ptr()
-        if timing:
-            ei0, ei1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            ei0.record(st)
-        _lib.check(lib.xtb_build_integrals(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
-                                           self.cn.data_ptr(), self.gamma.data_ptr(),
-                                           self.out.data_ptr(), s), "xtb_build_integrals")
-        if timing:
-            ei1.record(st)
-            self.int_events = (ei0, ei1)
+        fused = fused and not self.large
+        self.int_events = None
+        if not fused:
+            if timing:
+                ei0, ei1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                ei0.record(st)
+            _lib.check(lib.xtb_build_integrals(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(),
+                                               self.cn.data_ptr(), self.gamma.data_ptr(),
+                                               self.out.data_ptr(), s), "xtb_build_integrals")
+            if timing:
+                ei1.record(st)
+                self.int_events = (ei0, ei1)
         # a warm start needs the state a previous keep_state launch on these
         # buffers left (in stream order, so the launches need no other sync)
         if self._kept and opts.warm_extrapolate:
@@ -455,6 +466,12 @@ class Pipeline:
                                          self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state),
                                          self.work.data_ptr(), self.work.numel(), s),
                        "xtb_scf_large")
+        elif fused:
+            _lib.check(lib.xtb_singlepoint(b, p, pos, self.S.data_ptr(), self.H0.data_ptr(), self.cn.data_ptr(),
+                                           self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state),
+                                           self.dedcn.data_ptr() if forces else None,
+                                           self.grad.data_ptr() if forces else None, s),
+                       "xtb_singlepoint")
         else:
             _lib.check(lib.xtb_scf(b, p, self.S.data_ptr(), self.H0.data_ptr(),
                                    self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state), s),
@@ -464,7 +481,7 @@ class Pipeline:
             self.scf_events = (ev0, ev1)
         self._kept = bool(opts.warm_start) and not self.large
         self._have_w = bool(forces)
-        if forces:
+        if forces and not fused:
             self.run_forces(stream)
 
     def stage_results(self, stream=None) -> None:
@@ -643,13 +660,14 @@ class SplitPipeline:
         self.out = torch.zeros((L.B, _lib.OUT_STRIDE), dtype=torch.float64, device=self.device)
         self.grad = torch.zeros((int(L.nat.sum()), 3), dtype=torch.float64, device=self.device)
 
-    def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None, **kw) -> None:
+    def run(self, positions: torch.Tensor, opts: ScfOptions, forces: bool, stream=None, fused: bool = False,
+            **kw) -> None:
         if positions.shape != (int(self.L.nat.sum()), 3):
             raise ValueError("positions do not match the batch layout")
         st = stream if stream is not None else torch.cuda.current_stream(self.device)
         with torch.cuda.stream(st):
             for idx, atoms, p in self.parts:
-                p.run(positions.index_select(0, atoms).contiguous(), opts, forces=forces, stream=st)
+                p.run(positions.index_select(0, atoms).contiguous(), opts, forces=forces, stream=st, fused=fused)
                 self.out.index_copy_(0, idx, p.records())
                 if forces:
                     self.grad.index_copy_(0, atoms, p.gradient())

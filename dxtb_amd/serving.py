@@ -141,7 +141,8 @@ class RaggedServer:
                  edges=DEFAULT_EDGES, max_batch=DEFAULT_MAXB, hbm_budget_gb: float = 64.0,
                  elements=(1, 6, 7, 8), forces: bool = True, max_wait_s: float = 0.002,
                  wait_frac: float = 0.5, max_wait_cap_s: float = 0.1, xcd_rotate: bool = True,
-                 stage_results: bool = False, timing: bool = False):
+                 stage_results: bool = False, timing: bool = False, merge_above: int | None = 1200,
+                 merge_cap: int = 2400, merge_largest: bool = True, fused: bool = True):
         self.device = torch.device(device)
         self.opts = opts or ScfOptions()
         self.tables = SpeciesTables(elements)
@@ -182,6 +183,18 @@ class RaggedServer:
         # 46-139 ms at 4300/s against 11-19 ms with plain copies when the
         # launch is retired).
         self.stage_results = stage_results
+        # saturation: with more than `merge_above` requests waiting, a launch
+        # takes `merge_cap` of them whatever their class (the kernel starts
+        # the largest molecules first), so that launches are few and long and
+        # each one's tail overlaps the next one's start (None: class launches
+        # only)
+        self.merge_above = merge_above
+        self.merge_cap = merge_cap
+        # merged launches take the largest waiting molecules first (longest
+        # job first: the backlog's longest SCFs start early instead of
+        # running alone at its end) rather than the oldest
+        self.merge_largest = merge_largest
+        self.fused = fused
         self._xcd_next = 0
         self.stats = ServerStats()
         self.results: dict[int, Request] = {}
@@ -196,6 +209,10 @@ class RaggedServer:
         if k >= len(self.edges):
             raise ValueError(f"molecule with {req.nao} orbitals exceeds the largest size class")
         self.queues[k].append(req)
+
+    def submit_front(self, req: Request) -> None:
+        """Put a request taken for a launch back at the head of its class."""
+        self.queues[int(np.searchsorted(self.edges, req.nao))].appendleft(req)
 
     def pending(self) -> int:
         return sum(len(q) for q in self.queues)
@@ -224,20 +241,47 @@ class RaggedServer:
         t = self.class_time[k]
         return min(self.max_wait_cap, max(self.max_wait, self.wait_frac * t))
 
+    def _take_merged(self) -> list[Request]:
+        """The oldest merge_cap waiting requests of all classes (ties: the
+        larger class first), or with merge_largest the largest classes'
+        requests first (oldest first within a class)."""
+        take = []
+        if self.merge_largest:
+            for q in reversed(self.queues):
+                while q and len(take) < self.merge_cap:
+                    take.append(q.popleft())
+            return take
+        while len(take) < self.merge_cap:
+            best, best_t = None, None
+            for k, q in enumerate(self.queues):
+                if q and (best is None or q[0].arrival <= best_t):
+                    best, best_t = k, q[0].arrival
+            if best is None:
+                break
+            q = self.queues[best]
+            # the class's requests of that same arrival time, in one go
+            while q and q[0].arrival == best_t and len(take) < self.merge_cap:
+                take.append(q.popleft())
+        return take
+
     def _admit(self, now: float, eager: bool = False) -> bool:
         if not self.free_streams:
             return False
-        k = self._pick_class(now, eager)
-        if k is None:
-            return False
-        q = self.queues[k]
-        take = [q.popleft() for _ in range(min(self.max_batch[k], len(q)))]
+        if self.merge_above is not None and self.pending() > self.merge_above:
+            k = -1
+            take = self._take_merged()
+        else:
+            k = self._pick_class(now, eager)
+            if k is None:
+                return False
+            q = self.queues[k]
+            take = [q.popleft() for _ in range(min(self.max_batch[k], len(q)))]
         lay = BatchLayout([r.numbers for r in take], self.tables, padded=False,
                           chrg=np.zeros(len(take)))
         nbytes = 8 * lay.work_doubles()
         if self.inflight and self._inflight_bytes() + nbytes > self.budget:
             for r in reversed(take):
-                q.appendleft(r)
+                self.submit_front(r)
             self.stats.admission_stalls += 1
             return False
         stream = self.free_streams.popleft()
@@ -251,8 +295,11 @@ class RaggedServer:
             np.concatenate([r.positions for r in take], out=hpos.numpy().reshape(nat, 3))
             pos = hpos.view(nat, 3).to(self.device, non_blocking=True)
             # launches start on the XCD after the previous launch's last one
+            # integrals, SCF and forces in one launch: under a busy loop every
+            # launch waits for CUs, so a batch's stages as separate kernels
+            # held its stream idle between them
             pipe.run(pos, self.opts, forces=self.forces, stream=stream, timing=self.timing,
-                     wg_skip=self._xcd_next if self.xcd_rotate else 0)
+                     wg_skip=self._xcd_next if self.xcd_rotate else 0, fused=self.fused)
             self._xcd_next = (self._xcd_next + len(take)) % 8
             if self.stage_results:
                 pipe.stage_results(stream)
@@ -298,8 +345,9 @@ class RaggedServer:
             self.stats.latencies.append(now - r.arrival)
         self.stats.completed += len(f.reqs)
         self.stats.scf_busy_s += float(rec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
-        t = float(rec[:, _lib.OUT_T_TOTAL].mean()) * 1e-6
-        self.class_time[f.klass] = 0.8 * self.class_time[f.klass] + 0.2 * t
+        if f.klass >= 0:
+            t = float(rec[:, _lib.OUT_T_TOTAL].mean()) * 1e-6
+            self.class_time[f.klass] = 0.8 * self.class_time[f.klass] + 0.2 * t
         if self.timing:
             self.stats.spans += list(zip(rec[:, _lib.OUT_T_START], rec[:, _lib.OUT_T_TOTAL]))
             # host admit time, retire time, molecules, max n, device first start, last end (us)

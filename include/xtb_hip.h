@@ -106,6 +106,8 @@ extern "C" {
 #define XTB_OUT_T_MULL 26   /* Mulliken charges + ES2/ES3 potential, microseconds */
 #define XTB_OUT_N_MULL 27   /* P and S reads of those passes (1.5 for the last:
                                it also reads H0 for the orbital energies) */
+#define XTB_OUT_T_INTEGRALS 28  /* xtb_singlepoint: the molecule's integral pass, microseconds */
+#define XTB_OUT_T_FORCES 29     /* xtb_singlepoint: the molecule's gradient, microseconds */
 
 /* Ragged batch descriptor.  All pointers are device pointers. */
 typedef struct xtb_batch {
@@ -347,6 +349,25 @@ int32_t xtb_forces(const xtb_batch* batch, const xtb_params* par,
                    const double* positions, const double* cn,
                    const double* gamma, const xtb_scf_state* state,
                    double* dedcn, double* grad, void* stream);
+
+/* The single point of every molecule in ONE launch: each molecule's
+ * workgroup computes its CN, S, H0, gamma and repulsion energy (as
+ * xtb_build_integrals, the same results bit for bit), runs its SCF (as
+ * xtb_scf with the same opts and state) and, with opts->want_w, writes its
+ * dE/dCN to `dedcn` [nat_total] and its gradient dE/dR [3*nat_total] to
+ * `grad` (written, not added; per-atom sums in a fixed order, so the same
+ * bits on every run; xtb_forces agrees to rounding).  Replaces the chain
+ * upstream's singlepoint + forces_analytical run (calculators/types/
+ * energy.py:78-402, analytical.py:61-222) with one kernel per batch, so
+ * that under a busy serving loop no small integral or gradient launch
+ * waits for compute units between a batch's stages.  The record fields
+ * XTB_OUT_T_INTEGRALS and XTB_OUT_T_FORCES time those parts, T_TOTAL the
+ * whole molecule.  state.F is scratch after the SCF (the pair terms of the
+ * gradient).  dedcn and grad may be NULL without want_w. */
+int32_t xtb_singlepoint(const xtb_batch* batch, const xtb_params* par,
+                        const double* positions, double* S, double* H0, double* cn,
+                        double* gamma, const xtb_scf_opts* opts, const xtb_scf_state* state,
+                        double* dedcn, double* grad, void* stream);
 
 /* Atoms a batch may hold in xtb_forces_noalloc (its scratch: 8 bytes per
  * atom, 2 MiB per device, held until the process ends). */

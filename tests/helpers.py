@@ -10,9 +10,10 @@ TIGHT = dict(x_atol=1e-11, x_atol_max=1e-11, maxiter=300)
 TIGHT_ORACLE = {"x_atol": 1e-11, "x_atol_max": 1e-11, "maxiter": 300}
 
 
-def run_native(mols, padded=False, forces=False, chrg=None, spin=None, **opt):
+def run_native(mols, padded=False, forces=False, chrg=None, spin=None, fused=False, **opt):
     """mols: list of (numbers, positions[bohr]); chrg, spin: per molecule
-    (spin None: automatic, as upstream); returns the Pipeline."""
+    (spin None: automatic, as upstream); fused: one launch (xtb_singlepoint)
+    instead of integrals, SCF and forces; returns the Pipeline."""
     dev = torch.device("cuda:0")
     zs = np.unique(np.concatenate([z for z, _ in mols]))
     tb = SpeciesTables(zs)
@@ -22,7 +23,7 @@ def run_native(mols, padded=False, forces=False, chrg=None, spin=None, **opt):
     pipe = Pipeline(lay, DeviceTables(tb, dev), dev)
     pos = torch.tensor(np.concatenate([p for _, p in mols]), dtype=torch.float64, device=dev)
     o = ScfOptions(**{**TIGHT, **opt})
-    pipe.run(pos, o, forces=forces)
+    pipe.run(pos, o, forces=forces, fused=fused)
     torch.cuda.synchronize()
     return pipe
 

@@ -205,3 +205,45 @@ def test_negative_temperature_is_rejected():
     with pytest.raises(ValueError):
         ScfOptions(fermi_etemp=-1.0).to_struct(want_w=False, padded=False)
     ScfOptions(fermi_etemp=0.0).to_struct(want_w=False, padded=False)
+
+
+def test_serving_merged_launch_takes_the_oldest_requests():
+    """Saturation launches (merge_above): the oldest merge_cap waiting
+    requests whatever their class, the larger class first among requests of
+    one arrival time; a launch that does not fit the memory budget puts its
+    requests back at the head of their classes in their order."""
+    from collections import deque
+
+    from dxtb_amd.serving import DEFAULT_EDGES, RaggedServer, Request
+
+    srv = object.__new__(RaggedServer)
+    srv.edges = DEFAULT_EDGES
+    srv.queues = [deque() for _ in DEFAULT_EDGES]
+    srv.merge_cap = 5
+    srv.merge_largest = False
+    z = np.array([1, 1])
+    mk = lambda rid, nao, t: Request(rid, z, np.zeros((2, 3)), arrival=t, nao=nao)
+    first = [mk(0, 10, 0.0), mk(1, 20, 0.0), mk(2, 30, 0.0), mk(3, 150, 0.0), mk(4, 140, 0.0),
+             mk(5, 700, 0.0)]
+    later = [mk(6 + i, 60, 1.0) for i in range(4)]
+    for r in first + later:
+        srv.submit(r)
+    assert srv.pending() == 10
+    take = srv._take_merged()
+    # class 5 (700), class 2 (150, 140), then class 0 in arrival order
+    assert [r.rid for r in take] == [5, 3, 4, 0, 1]
+    for r in reversed(take):
+        srv.submit_front(r)
+    assert [r.rid for r in srv.queues[0]] == [0, 1, 2]
+    assert [r.rid for r in srv.queues[2]] == [3, 4]
+    take = srv._take_merged()
+    assert [r.rid for r in take] == [5, 3, 4, 0, 1]
+    take = srv._take_merged()
+    assert [r.rid for r in take] == [2, 6, 7, 8, 9]
+    assert srv.pending() == 0 and srv._take_merged() == []
+    # largest first: class 5, class 2, class 1 (later arrivals), class 0
+    srv.merge_largest = True
+    for r in first + later:
+        srv.submit(r)
+    assert [r.rid for r in srv._take_merged()] == [5, 3, 4, 6, 7]
+    assert [r.rid for r in srv._take_merged()] == [8, 9, 0, 1, 2]
