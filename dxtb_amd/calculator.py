@@ -271,7 +271,9 @@ class GFN1Calculator:
         orbitals = orbitals or opts.orbital_energies
         if orbitals != opts.orbital_energies:
             opts = dataclasses.replace(opts, orbital_energies=True)
-        pipe.run(flat, opts, forces=forces)
+        # one launch per batch (integral pass, SCF and forces per molecule;
+        # the gradient's sums in a fixed order, so the same bits every run)
+        pipe.run(flat, opts, forces=forces, fused=True)
         rec = pipe.records().clone()
         status = rec[:, _lib.OUT_STATUS].to(torch.int64).cpu().numpy()
         self._raise_on_status(status)
