@@ -325,8 +325,12 @@ def roofline_pass(stream, device, opts, start: int = ROOFLINE_START, count: int 
     # workgroup owns one CU, whose share of the HBM peak is peak / 256
     nf, tf = rec[:, _lib.OUT_N_FOCK], rec[:, _lib.OUT_T_FOCK]
     nm, tmu = rec[:, _lib.OUT_N_MULL], rec[:, _lib.OUT_T_MULL]
+    steps = rec[:, _lib.OUT_ITER] + 1.0
     fock_bytes = float((24.0 * nf * n2).sum())
-    mull_bytes = float((16.0 * nm * n2 + 8.0 * (nm - 0.5) * m2).sum())
+    # the last step's Mulliken pass reads P, S and H0 (nm = 1.5 units of
+    # 16 n^2); the other steps take q from the density product's tiles and
+    # read only gamma here
+    mull_bytes = float((16.0 * nm * n2 + 8.0 * steps * m2).sum())
     fock_gbs = fock_bytes / (float(tf.sum()) * 1e-6) / 1e9
     mull_gbs = mull_bytes / (float(tmu.sum()) * 1e-6) / 1e9
     share = HBM_PEAK_GBS / NCU
@@ -340,12 +344,13 @@ def roofline_pass(stream, device, opts, start: int = ROOFLINE_START, count: int 
                        "frac_of_cu_share": round(fock_gbs / share, 4),
                        "share_of_scf_time": round(float(tf.sum()) * 1e-6 / busy, 4),
                        "bytes_rule": "24 n^2 per pass (read H0, S; write F)"},
-        "mulliken_potential": {"passes": int(round(float((nm - 0.5).sum()))),
+        "mulliken_potential": {"passes": int(round(float(steps.sum()))),
                                "workgroup_s": round(float(tmu.sum()) * 1e-6, 4),
                                "gbs_per_workgroup": round(mull_gbs, 2),
                                "frac_of_cu_share": round(mull_gbs / share, 4),
                                "share_of_scf_time": round(float(tmu.sum()) * 1e-6 / busy, 4),
-                               "bytes_rule": "16 n^2 (P, S; +8 n^2 H0 in the last pass) + 8 m^2 (gamma)"},
+                               "bytes_rule": "8 m^2 (gamma) per step; + 24 n^2 (P, S, H0) in the last step, "
+                                             "the others take q from the density product's tiles"},
         "cu_share_gbs": share,
     }
     stages["forces_ms"] = round(forces_ms, 3)

@@ -541,19 +541,20 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         // C_vir += K^T R
         gemm<true, false>(m, n, k, 1.0, Kb, m, cur, n, 1.0, Cv, n, sh.big);
         PD_MARK(sh, 7, tm);
-        // C_occ <- cos(X) C_occ - sinc(X) Z
-        wg_axpby((size_t)k * n, cc[J], Ct, -ss[J], Z, cur);
+        // C_occ <- cos(X) C_occ - sinc(X) Z.  The last Horner term is written
+        // over C_occ itself: each element of C_occ is read (as E1) and
+        // written by the same thread, and no product takes C_occ as an
+        // operand, so no copy pass follows
+        wg_axpby((size_t)k * n, cc[J], Ct, -ss[J], Z, J == 0 ? Ct : cur);
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
-            gemm_epi(k, n, k, X, k, cur, n, nxt, n, Ct, cc[j], Z, -ss[j], sh.big);
+            gemm_epi(k, n, k, X, k, cur, n, j == 0 ? Ct : nxt, n, Ct, cc[j], Z, -ss[j], sh.big);
             PD_MARK(sh, 5, tm);
             glbf* t = cur;
             cur = nxt;
             nxt = t;
         }
-        wg_axpby((size_t)k * n, 1.0, cur, 0.0, nullptr, Ct);
-        __syncthreads();
         PD_MARK(sh, 6, tm);
 #ifdef XTB_PD_TIMERS
         sh.ptacc[9] += (double)J;
@@ -567,7 +568,8 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
 
 // ---------------------------------------------------------------------------
 // one SCF map v_in -> v_out (scf/base.py:651-675 iterate_potential);
-// leaves P, q, e_orb, sorted e/occupations and shell/atom charges behind.
+// leaves q, sorted e/occupations and shell/atom charges behind, and in the
+// last step (`last`) also P and e_orb (the other steps never store P).
 // ---------------------------------------------------------------------------
 __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_batch& b,
                         const xtb_params& par, const xtb_scf_opts& o, const glbf* vin,
@@ -708,7 +710,14 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
         }
         __syncthreads();
     }
-    gemm_sym<true, false>(n, kocc, 1.0, perm ? M.F : M.Ct, n, M.T, n, M.P, n, sh.big);
+    if (last) {
+        gemm_sym<true, false>(n, kocc, 1.0, perm ? M.F : M.Ct, n, M.T, n, M.P, n, sh.big);
+    } else {
+        // an intermediate step needs the charges, not P: the Mulliken sums
+        // a_i = sum_j P_ij S_ij come from the density product's tiles (P is
+        // never stored; its buffer holds the tiles' partial sums), into q
+        gemm_sym_mull<true, false>(n, kocc, perm ? M.F : M.Ct, n, M.T, n, M.S, M.nao, M.P, M.q, sh.big);
+    }
 #ifdef XTB_PD_TIMERS
     sh.ptacc[8] += now_us() - t0;
 #endif
@@ -720,7 +729,9 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     const double tm0 = now_us();
     // Mulliken charges and orbital energies (scf/base.py:764-815), one wave
     // per row; the orbital energies only enter the final energy (last step)
-    {
+    if (!last) {
+        for (int i = threadIdx.x; i < n; i += NT) M.q[i] = (i < M.nao) ? b.n0[M.vec0 + i] - M.q[i] : 0.0;
+    } else {
         const int w = threadIdx.x / WAVE, l = threadIdx.x % WAVE;
         for (int i0 = w * ROWS_PER_WAVE; i0 < n; i0 += NWAVE * ROWS_PER_WAVE) {
             int ii[ROWS_PER_WAVE];
@@ -826,7 +837,10 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     }
     __syncthreads();
     sh.tacc[10] += now_us() - tm0;
-    sh.tacc[11] += last ? 1.5 : 1.0;  // the last step also reads H0 (orbital energies)
+    // (P, S) reads of this stage in units of 16 n^2 bytes: the last step
+    // reads P, S and H0 (orbital energies); the others form q inside the
+    // density product (timed with it) and only add the gamma products here
+    sh.tacc[11] += last ? 1.5 : 0.0;
     return sw < 0 ? 0 : sw;
 }
 

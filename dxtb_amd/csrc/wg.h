@@ -141,14 +141,24 @@ __device__ inline v4d v4zero() {
 #endif
 
 // One TM x TN output tile at (m0, n0).  With EPI, C = alpha op(A) op(B) + s1 E1 + s2 E2 (E1, E2 with C's leading
-// dimension; beta unused): an elementwise update fused into the epilogue.
-template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false>
+// dimension; beta unused): an elementwise update fused into the epilogue.  C may
+// be E1 or E2 (each element is read and then written by the same thread, once);
+// A and B must not overlap C.
+//
+// With MULL (C not written): the tile's elements x = alpha (op(A) op(B))_ij
+// E1_ij over rows and columns below `nao` are summed per row (over the
+// tile's columns) and per column (over its rows); part[0..prow) gets the
+// first prow row sums and, with pcol, part[TM..TM+TN) the column sums.
+// Lanes reduce by xor butterflies (the same bits in every lane), waves
+// through LDS in index order: the sums have the same bits on every run.
+template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false, bool MULL = false>
 __device__ __forceinline__ void gemm_tile(int m0, int n0, int M, int N, int K, double alpha,
                                           const glbf* __restrict__ A, int lda,
                                           const glbf* __restrict__ B, int ldb, double beta,
-                                          glbf* __restrict__ C, int ldc, ldsf* lds,
-                                          const glbf* __restrict__ E1, const glbf* __restrict__ E2,
-                                          double s1, double s2) {
+                                          glbf* C, int ldc, ldsf* lds,
+                                          const glbf* E1, const glbf* E2,
+                                          double s1, double s2, int nao = 0, glbf* part = nullptr,
+                                          int prow = 0, bool pcol = false) {
     static_assert(NT == 512, "wave layout assumes 8 waves");
     constexpr int TM = 32 * FM, TN = 64 * FN;
     constexpr int LDA_S = TM + 1, LDB_S = TN + 1;
@@ -284,6 +294,52 @@ __device__ __forceinline__ void gemm_tile(int m0, int n0, int M, int N, int K, d
             __syncthreads();
         }
     }
+    if constexpr (MULL) {
+        double rs[FM][4], cs[FN];
+#pragma unroll
+        for (int j = 0; j < FN; ++j) cs[j] = 0.0;
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                rs[i][r] = 0.0;
+                const int gm = m0 + wm * (TM / 2) + i * 16 + lg + 4 * r;
+#pragma unroll
+                for (int j = 0; j < FN; ++j) {
+                    const int gn = n0 + wn * (TN / 4) + j * 16 + li;
+                    const double x =
+                        (gm < nao && gn < nao) ? alpha * acc[i][j][r] * E1[(size_t)gm * ldc + gn] : 0.0;
+                    rs[i][r] += x;
+                    cs[j] += x;
+                }
+            }
+#pragma unroll
+        for (int i = 0; i < FM; ++i)
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) rs[i][r] += __shfl_xor(rs[i][r], o, WAVE);
+#pragma unroll
+        for (int j = 0; j < FN; ++j)
+#pragma unroll
+            for (int o = 16; o < WAVE; o <<= 1) cs[j] += __shfl_xor(cs[j], o, WAVE);
+        ldsf* rred = lds;           // [4 column waves][TM]
+        ldsf* cred = lds + 4 * TM;  // [2 row waves][TN]
+        if (li == 0)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) rred[wn * TM + wm * (TM / 2) + i * 16 + lg + 4 * r] = rs[i][r];
+        if (lg == 0)
+#pragma unroll
+            for (int j = 0; j < FN; ++j) cred[wm * TN + wn * (TN / 4) + j * 16 + li] = cs[j];
+        __syncthreads();
+        for (int e = t; e < prow; e += NT) part[e] = ((rred[e] + rred[TM + e]) + rred[2 * TM + e]) + rred[3 * TM + e];
+        if (pcol)
+            for (int e = t; e < TN; e += NT) part[TM + e] = cred[e] + cred[TN + e];
+        __syncthreads();  // the LDS stages of the next tile reuse these rows
+        return;
+    }
 #pragma unroll
     for (int i = 0; i < FM; ++i)
 #pragma unroll
@@ -307,9 +363,9 @@ __device__ __forceinline__ void gemm_tile(int m0, int n0, int M, int N, int K, d
 template <bool TA, bool TB, int FM, int FN, int KP, bool EPI = false>
 __device__ __noinline__ void wg_gemm_mfma(int M, int N, int K, double alpha, const glbf* __restrict__ A,
                                           int lda, const glbf* __restrict__ B, int ldb, double beta,
-                                          glbf* __restrict__ C, int ldc, ldsf* lds, int upper,
-                                          const glbf* __restrict__ E1 = nullptr,
-                                          const glbf* __restrict__ E2 = nullptr, double s1 = 0.0,
+                                          glbf* C, int ldc, ldsf* lds, int upper,
+                                          const glbf* E1 = nullptr,
+                                          const glbf* E2 = nullptr, double s1 = 0.0,
                                           double s2 = 0.0) {
     constexpr int TM = 32 * FM, TN = 64 * FN;
     // arguments of an out-of-line call arrive in vector registers; they are
@@ -412,6 +468,58 @@ __device__ inline void gemm_sym(int n, int K, double alpha, const glbf* A, int l
             }
             __syncthreads();
         }
+}
+
+// Mulliken sums a_i = sum_{j < nao} P_ij S_ij (0 for i >= nao) of the
+// symmetric n x n product P = op(A) op(B), without forming P: the tiles on
+// and above the diagonal on matrix cores, each reducing P o S in its
+// epilogue per row and per column (a tile above the diagonal stands, by its
+// column sums, for the rows of its mirror image); the partials are added per
+// row in a fixed order.  Square tiles as gemm_sym.  `part`: scratch of
+// 2 T tm^2 doubles (T the tile edge, tm = ceil(n / T)) when n > T; one tile
+// writes its row sums straight to `out`.  S with leading dimension n.
+template <bool TA, bool TB, int FM>
+__device__ __noinline__ void wg_gemm_sym_mull(int n, int K, const glbf* __restrict__ A, int lda,
+                                              const glbf* __restrict__ B, int ldb, const glbf* S, int nao,
+                                              glbf* part, glbf* out, ldsf* lds) {
+    constexpr int FN = FM / 2;
+    constexpr int T = 32 * FM;
+    static_assert(T == 64 * FN, "square tiles");
+    n = __builtin_amdgcn_readfirstlane(n);
+    K = __builtin_amdgcn_readfirstlane(K);
+    lda = __builtin_amdgcn_readfirstlane(lda);
+    ldb = __builtin_amdgcn_readfirstlane(ldb);
+    nao = __builtin_amdgcn_readfirstlane(nao);
+    const int tm = (n + T - 1) / T;
+    if (tm == 1) {
+        gemm_tile<TA, TB, FM, FN, GEMM_KP, false, true>(0, 0, n, n, K, 1.0, A, lda, B, ldb, 0.0, nullptr, n, lds,
+                                                         S, nullptr, 0.0, 0.0, nao, out, n, false);
+        __syncthreads();
+        return;
+    }
+    for (int tile = 0; tile < tm * tm; ++tile) {
+        const int I = tile / tm, J = tile % tm;
+        if (I > J) continue;
+        gemm_tile<TA, TB, FM, FN, GEMM_KP, false, true>(I * T, J * T, n, n, K, 1.0, A, lda, B, ldb, 0.0, nullptr, n,
+                                                         lds, S, nullptr, 0.0, 0.0, nao,
+                                                         part + (size_t)tile * 2 * T, T, I != J);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += NT) {
+        const int I = i / T, r = i - I * T;
+        double a = 0.0;
+        for (int J = I; J < tm; ++J) a += part[(size_t)(I * tm + J) * 2 * T + r];
+        for (int J = 0; J < I; ++J) a += part[(size_t)(J * tm + I) * 2 * T + T + r];
+        out[i] = a;
+    }
+    __syncthreads();
+}
+
+template <bool TA, bool TB>
+__device__ inline void gemm_sym_mull(int n, int K, const glbf* A, int lda, const glbf* B, int ldb, const glbf* S,
+                                     int nao, glbf* part, glbf* out, ldsf* lds) {
+    if (n <= 64) wg_gemm_sym_mull<TA, TB, 2>(n, K, A, lda, B, ldb, S, nao, part, out, lds);
+    else wg_gemm_sym_mull<TA, TB, 4>(n, K, A, lda, B, ldb, S, nao, part, out, lds);
 }
 
 // ---------------------------------------------------------------------------

@@ -104,8 +104,10 @@ extern "C" {
 #define XTB_OUT_T_FOCK 24   /* Fock builds F = H0 - S o (v_i + v_j)/2, microseconds */
 #define XTB_OUT_N_FOCK 25   /* Fock builds done (each reads H0 and S, writes F) */
 #define XTB_OUT_T_MULL 26   /* Mulliken charges + ES2/ES3 potential, microseconds */
-#define XTB_OUT_N_MULL 27   /* P and S reads of those passes (1.5 for the last:
-                               it also reads H0 for the orbital energies) */
+#define XTB_OUT_N_MULL 27   /* P, S (and H0) reads of those passes in units of
+                               16 n^2 bytes: 1.5 for the last step (orbital
+                               energies too); the other steps take the charges
+                               from the density product and count 0 */
 #define XTB_OUT_T_INTEGRALS 28  /* xtb_singlepoint: the molecule's integral pass, microseconds */
 #define XTB_OUT_T_FORCES 29     /* xtb_singlepoint: the molecule's gradient, microseconds */
 
