@@ -215,39 +215,58 @@ __device__ __noinline__ bool anderson(Mixer& mx, const Mol& M, const SH& sh, con
     int nn = 0;
     if (mx.step > gen || !mx.soft) nn = (mx.step - 1 < gen) ? mx.step - 1 : gen;  // soft start
     if (nn > 0) {
-        // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn
+        // a_hj = (f0 - f_h).(f0 - f_j), b_h = (f0 - f_h).f0 ; h, j = 1..nn.
+        // One pass per h over the elements with at most MIX_MAXG + 1 sums
+        // live; each sum is reduced at once (wave butterfly, then the waves'
+        // partials in LDS added in wave order), so no per-thread array of all
+        // NACC sums stays live (it spilled to scratch on every call)
         constexpr int NA = MIX_MAXG * (MIX_MAXG + 1) / 2, NACC = NA + MIX_MAXG;
-        double acc[NACC];
+        static_assert(NACC * NWAVE + NACC <= 256, "reduction scratch");
+        const int wv = threadIdx.x / WAVE, ln = threadIdx.x % WAVE;
+        ldsf* part = sh.red;                 // [NACC][NWAVE]
+        ldsf* tot = sh.red + NACC * NWAVE;   // [NACC]
+        int k0 = 0;
 #pragma unroll
-        for (int k = 0; k < NACC; ++k) acc[k] = 0.0;
-        for (int i = threadIdx.x; i < n; i += NT) {
-            double d[MIX_MAXG];
-            const double fi = f0[i];
+        for (int h = 0; h < MIX_MAXG; ++h) {
+            double sj[MIX_MAXG + 1];
 #pragma unroll
-            for (int h = 0; h < MIX_MAXG; ++h) d[h] = (h < nn) ? fi - Fh(h + 1)[i] : 0.0;
-            int k = 0;
+            for (int j = 0; j <= MIX_MAXG; ++j) sj[j] = 0.0;
+            if (h < nn) {
+                for (int i = threadIdx.x; i < n; i += NT) {
+                    const double fi = f0[i];
+                    const double dh = fi - Fh(h + 1)[i];
 #pragma unroll
-            for (int h = 0; h < MIX_MAXG; ++h)
+                    for (int j = h; j < MIX_MAXG; ++j)
+                        if (j < nn) sj[j] = fma(dh, fi - Fh(j + 1)[i], sj[j]);
+                    sj[MIX_MAXG] = fma(dh, fi, sj[MIX_MAXG]);
+                }
+            }
 #pragma unroll
-                for (int j = h; j < MIX_MAXG; ++j) acc[k++] += d[h] * d[j];
-#pragma unroll
-            for (int h = 0; h < MIX_MAXG; ++h) acc[NA + h] += d[h] * fi;
+            for (int j = h; j <= MIX_MAXG; ++j) {
+                const double v = wave_sum(sj[j]);
+                const int e = j < MIX_MAXG ? k0 + (j - h) : NA + h;
+                if (ln == 0) part[e * NWAVE + wv] = v;
+            }
+            k0 += MIX_MAXG - h;
         }
-        static_assert(NACC * NWAVE <= 256, "one pass of the reduction scratch");
-        wg_sum_n<NACC>(acc, sh.red);
+        __syncthreads();
+        if (threadIdx.x < NACC) {
+            double v = 0.0;
+            for (int x = 0; x < NWAVE; ++x) v += part[threadIdx.x * NWAVE + x];
+            tot[threadIdx.x] = v;
+        }
+        __syncthreads();
         // solve (a * diag(1 + off^2)) theta = b by Gaussian elimination with
-        // partial pivoting; one thread, system held in LDS
+        // partial pivoting; one thread, system held in LDS (over the partials)
         constexpr int W = MIX_MAXG + 1;
         ldsf* a = sh.red;                  // [MIX_MAXG][W]
         ldsf* th = sh.red + MIX_MAXG * W;  // [MIX_MAXG]
+        static_assert(MIX_MAXG * W + MIX_MAXG <= NACC * NWAVE, "solver rows below the totals");
         if (threadIdx.x == 0) {
             int k = 0;
-#pragma unroll
             for (int h = 0; h < MIX_MAXG; ++h)
-#pragma unroll
-                for (int j = h; j < MIX_MAXG; ++j) { a[h * W + j] = acc[k]; a[j * W + h] = acc[k]; ++k; }
-#pragma unroll
-            for (int h = 0; h < MIX_MAXG; ++h) a[h * W + MIX_MAXG] = acc[NA + h];
+                for (int j = h; j < MIX_MAXG; ++j) { a[h * W + j] = tot[k]; a[j * W + h] = tot[k]; ++k; }
+            for (int h = 0; h < MIX_MAXG; ++h) a[h * W + MIX_MAXG] = tot[NA + h];
             for (int h = 0; h < nn; ++h) a[h * W + h] *= 1.0 + o.diag_offset * o.diag_offset;
             for (int c = 0; c < nn; ++c) {
                 int piv = c;
