@@ -396,6 +396,18 @@ __device__ inline void fock_build(const Mol& M, const glbf* vin) {
 // Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the two
 // Horner accumulators (k x n each, k <= n/2).
 // ---------------------------------------------------------------------------
+// Coefficient of term j of the series in X = K K^T used by the rotation:
+// which 0: cos, (-1)^j / (2j)!; 1: sinc, (-1)^j / (2j+1)!; 2: phi,
+// -(-1)^j / (2j+2)!  ((2j)! by the running product (2t-1)(2t), t <= j)
+__device__ inline double horner_coef(int j, int which) {
+    double f = 1.0;
+    for (int t = 1; t <= j; ++t) f *= (2.0 * t - 1.0) * (2.0 * t);
+    const double sg = (j & 1) ? -1.0 : 1.0;
+    if (which == 0) return sg / f;
+    if (which == 1) return sg / (f * (2.0 * j + 1.0));
+    return -sg / (f * (2.0 * j + 1.0) * (2.0 * j + 2.0));
+}
+
 __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_scf_opts& o, int k,
                                     double loose, const glbf* vin) {
     const int n = M.n, m = n - k;
@@ -511,27 +523,21 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
             term *= xf / ((2.0 * J + 1.0) * (2.0 * J + 2.0));
             if (J > 8) return fail;  // F and Ct untouched in this pass
         }
-        double cc[9], ss[9], ph[9];
-        {
-            double f = 1.0;  // (2j)!
-            for (int j = 0; j <= 8; ++j) {
-                if (j > 0) f *= (2.0 * j - 1.0) * (2.0 * j);
-                const double sg = (j & 1) ? -1.0 : 1.0;
-                cc[j] = sg / f;
-                ss[j] = sg / (f * (2.0 * j + 1.0));
-                ph[j] = -sg / (f * (2.0 * j + 1.0) * (2.0 * j + 2.0));
-            }
-        }
+        // series coefficients of term j (computed where used: arrays indexed
+        // by the run-time j would live in scratch memory)
+        auto cc = [](int j) { return horner_coef(j, 0); };
+        auto ss = [](int j) { return horner_coef(j, 1); };
+        auto ph = [](int j) { return horner_coef(j, 2); };
         // Horner steps acc <- X acc + a_j C_occ + b_j Z, the update fused into
         // the GEMM epilogue; the two k x n halves of F alternate
         glbf* cur = M.F;
         glbf* nxt = M.F + (size_t)k * n;
         // R = sinc(X) C_occ + phi(X) Z
-        wg_axpby((size_t)k * n, ss[J], Ct, ph[J], Z, cur);
+        wg_axpby((size_t)k * n, ss(J), Ct, ph(J), Z, cur);
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
-            gemm_epi(k, n, k, X, k, cur, n, nxt, n, Ct, ss[j], Z, ph[j], sh.big);
+            gemm_epi(k, n, k, X, k, cur, n, nxt, n, Ct, ss(j), Z, ph(j), sh.big);
             PD_MARK(sh, 5, tm);
             glbf* t = cur;
             cur = nxt;
@@ -545,11 +551,11 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         // over C_occ itself: each element of C_occ is read (as E1) and
         // written by the same thread, and no product takes C_occ as an
         // operand, so no copy pass follows
-        wg_axpby((size_t)k * n, cc[J], Ct, -ss[J], Z, J == 0 ? Ct : cur);
+        wg_axpby((size_t)k * n, cc(J), Ct, -ss(J), Z, J == 0 ? Ct : cur);
         __syncthreads();
         for (int j = J - 1; j >= 0; --j) {
             PD_MARK(sh, 6, tm);
-            gemm_epi(k, n, k, X, k, cur, n, j == 0 ? Ct : nxt, n, Ct, cc[j], Z, -ss[j], sh.big);
+            gemm_epi(k, n, k, X, k, cur, n, j == 0 ? Ct : nxt, n, Ct, cc(j), Z, -ss(j), sh.big);
             PD_MARK(sh, 5, tm);
             glbf* t = cur;
             cur = nxt;
