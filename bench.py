@@ -658,9 +658,10 @@ def main() -> int:
             "padded_batch": pb0,
             "padded_dropin_by_batch": {str(k): round(v, 2) for k, v in padded.items()},
             "ragged_over_padded_by_batch": {str(k): round(value / v, 2) for k, v in padded.items()},
-            "padded_dropin_eigen": "padded layout: every step a full Jacobi eigensolve of the batch-maximum "
-                                   "n (no occupation-class or perturbative steps, as those need each "
-                                   "molecule's own gap); warm-started from the previous step's orbitals",
+            "padded_dropin_eigen": "padded layout: every matrix at the batch-maximum n; the same eigen "
+                                   "options as the ragged loop (occupation-class and perturbative steps "
+                                   "where a molecule's frontier gap allows, the padding levels at energy 0 "
+                                   "taking part in its spectrum); one fused launch per batch",
             "latency_requests": a.latency_requests,
             "latency_rate_per_s": head["rate_per_s"], "p50_ms": head["p50_ms"],
             "p99_ms": head["p99_ms"], "latency_curve": curve, "nonconverged": bad,

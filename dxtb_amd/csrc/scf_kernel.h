@@ -588,9 +588,13 @@ __device__ __noinline__ int scf_step(const Mol& M, const Shared& sh, const xtb_b
     // step's spectrum (sh.skey, ascending; the rows of Ct are in that order),
     // only the coupling between doubly, singly and un-occupied orbitals is
     // removed: that fixes the density of 0/1 occupations, whose Fermi tails
-    // at such gaps are below 1/(1 + e^(bd_gap_kt/2)).
+    // at such gaps are below 1/(1 + e^(bd_gap_kt/2)).  The padding orbitals
+    // of a padded layout (unit overlap diagonal, zero H0 and potential) are
+    // exact eigenvectors at energy 0 with no coupling to any other orbital;
+    // they sit in whichever class the sorted spectrum puts them, and the gap
+    // test below covers them like any other level.
     int cb_lo = -1, cb_hi = -1;
-    if ((allow_bd || final_step) && o.bd_gap_kt > 0.0 && !o.padded) {
+    if ((allow_bd || final_step) && o.bd_gap_kt > 0.0) {
         const double na = M.nel_a, nb = M.nel_b;
         const int ia = (int)rint(na), ib = (int)rint(nb);
         bool ok = fabs(na - ia) < 1e-12 && fabs(nb - ib) < 1e-12 && ib <= ia;

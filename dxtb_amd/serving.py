@@ -460,7 +460,7 @@ def padded_dropin(reqs: list[Request], batch: int, device="cuda:0", opts: ScfOpt
         lay = BatchLayout([n[n > 0] for n in nums], tables, padded=True, chrg=np.zeros(len(chunk)))
         pipe = Pipeline(lay, dt, dev)
         p = torch.as_tensor(pos[mask], dtype=torch.float64, device=dev)
-        pipe.run(p, opts, forces=forces)
+        pipe.run(p, opts, forces=forces, fused=True)
         e = pipe.energies().cpu().numpy()
         if forces:
             pipe.gradient().cpu()
