@@ -992,7 +992,8 @@ int pd(Ctx& c, int k, double loose, const double* vin) {
     c.refresh = refresh ? 1 : c.refresh + 1;
     const int rows = refresh ? n : k;
     int ret = loose > 0.0 ? 1 : -1;
-    for (int pass = 0; pass < 2; ++pass) {
+    const int npass = loose > 0.0 ? PD_INEXACT_PASSES : PD_EXACT_PASSES;
+    for (int pass = 0; pass < npass; ++pass) {
         if (pass > 0) fock(c, vin);
         gemm_t<false, false, false>(c, rows, n, n, 1.0, c.Ct, n, c.F, n, 0.0, c.T, n, false);  // C F
         gemm_t<false, true, false>(c, k, m, n, 1.0, c.T, n, Cv, n, 0.0, Kb, m, false);        // A_ov

@@ -38,6 +38,17 @@ struct Shared {
 // device wall clock (constant 100 MHz) in microseconds
 __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 
+// Passes of the perturbative occupied-virtual rotation: an inexact SCF step
+// stops after two; the final (exact) step runs up to PD_EXACT_PASSES before
+// it falls back to a full eigensolve.  The passes converge linearly once the
+// occupied and virtual blocks are far from diagonal (the amplitudes use
+// their diagonals only): on 2400 molecules of the serving stream at the
+// default convergence threshold the final step needs 2 passes for 73 %, 3
+// for 18 %, 4-6 for 5 % of them, while a full eigensolve of a 500-orbital
+// molecule costs as much as ~15 passes.
+constexpr int PD_INEXACT_PASSES = 2;
+constexpr int PD_EXACT_PASSES = 6;
+
 // ---------------------------------------------------------------------------
 // Diagonal blocks of the blocked Cholesky orthogonaliser (one wave, 32 x 32
 // in LDS, row-major).  chol_block_factor: D <- its lower Cholesky factor in

@@ -390,8 +390,8 @@ __device__ inline void fock_build(const Mol& M, const glbf* vin) {
 // ask for the Jacobi eigensolve with Ct and F untouched, -1 to ask for it
 // after F was used as scratch (the caller rebuilds F).  Jacobi is asked for
 // when an amplitude exceeds pd_kmax, a denominator is not positive, k > n/2,
-// a series would need more than 9 terms, or (exact calls, loose = 0) two
-// passes leave a coupling above the tolerance.  On return 1, sh.ev holds the
+// a series would need more than 9 terms, or (exact calls, loose = 0)
+// PD_EXACT_PASSES passes leave a coupling above the tolerance.  On return 1, sh.ev holds the
 // diagonal of C F C^T in the rows' order (occupied rows: of the last pass).
 // Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the two
 // Horner accumulators (k x n each, k <= n/2).
@@ -431,7 +431,8 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
     const int rows = refresh ? n : k;
     double tm = now_us();
     (void)tm;
-    for (int pass = 0; pass < 2; ++pass) {
+    const int npass = loose > 0.0 ? PD_INEXACT_PASSES : PD_EXACT_PASSES;
+    for (int pass = 0; pass < npass; ++pass) {
         if (pass > 0) fock_build(M, vin);
         // T = C F (rows: all, or the occupied ones); A_ov = T_occ C_vir^T
         gemm<false, false>(rows, n, n, 1.0, Ct, n, M.F, n, 0.0, M.T, n, sh.big);
@@ -567,8 +568,8 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
 #endif
         if (kmax * aov <= tol) return 1;  // second-order remainder below the tolerance
     }
-    // two passes: good enough for an inexact step; an exact one (loose = 0)
-    // asks for Jacobi on the rotated orbitals instead
+    // the passes spent: good enough for an inexact step; an exact one
+    // (loose = 0) asks for Jacobi on the rotated orbitals instead
     return loose > 0.0 ? 1 : -1;
 }
 
