@@ -49,8 +49,10 @@ _DISP_WARNED = False
 # a single molecule with more orbitals than this runs its SCF on the whole
 # GPU (xtb_scf_large); batches use the per-molecule kernel, except for their
 # molecules above its limit (WG_MAX_N orbitals), which take the whole GPU
-# one after the other
-LARGE_N = 600
+# one after the other.  Energy + forces of one molecule alone, one CU /
+# whole GPU: n = 100: 4.6 / 7.0 ms, 154: 12.9 / 9.8, 236: 15.0 / 12.0,
+# 350: 38 / 18, 460: 70 / 20 (chains; water clusters alike)
+LARGE_N = 192
 
 
 def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:

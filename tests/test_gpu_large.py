@@ -175,3 +175,32 @@ def test_batch_with_molecule_above_kernel_limit(fx, padded):
         else:
             ref = o["emo"].numpy()
         assert np.abs(eb - ref).max() < 1e-8
+
+
+def test_calculator_routes_a_single_molecule_by_size():
+    """GFN1Calculator sends one molecule above LARGE_N orbitals to the
+    whole-GPU path and a smaller one to the per-molecule kernel; both give
+    what the per-molecule kernel gives for the same molecule (energy,
+    forces, charges), and the batch of the two goes to the kernel."""
+    from dxtb_amd import GFN1Calculator
+    from dxtb_amd.calculator import LARGE_N
+
+    dev = torch.device("cuda:0")
+    rng = np.random.default_rng(11)
+    mols = [W.chain(rng, 30), W.chain(rng, 12)]  # n = 236 and 100 orbitals
+    ref = run_native(mols, forces=True, **TIGHT)
+    opts = {"maxiter": 300, "x_atol": 1e-10, "x_atol_max": 1e-10, "exclude": ["disp"]}
+    for k, (z, p) in enumerate(mols):
+        print(f"case: calculator, one molecule of {len(z)} atoms", flush=True)
+        calc = GFN1Calculator(torch.tensor(z, device=dev), opts=opts, dtype=torch.float64, device=dev)
+        pos = torch.tensor(p, dtype=torch.float64, device=dev, requires_grad=True)
+        e = calc.get_energy(pos)
+        f = calc.get_forces(pos)
+        q = calc.get_charges(pos.detach())
+        n = int(ref.L.n[k])
+        assert calc._pipe.large == (n > LARGE_N)
+        assert abs(e.item() - ref.energies()[k].item()) < 1e-9
+        a0, na = int(ref.L.atom0[k]), len(z)
+        assert (f + ref.gradient()[a0:a0 + na]).abs().max().item() < 1e-8
+        assert (q - ref.orbital_charges(k)).abs().max().item() < 1e-8
+    assert int(ref.L.n[0]) > LARGE_N >= int(ref.L.n[1])
