@@ -991,7 +991,7 @@ int pd(Ctx& c, int k, double loose, const double* vin) {
     const bool refresh = c.refresh == 0 || c.refresh >= 4;
     c.refresh = refresh ? 1 : c.refresh + 1;
     const int rows = refresh ? n : k;
-    int ret = loose > 0.0 ? 1 : -1;
+    int ret = 1;
     const int npass = loose > 0.0 ? PD_INEXACT_PASSES : PD_EXACT_PASSES;
     for (int pass = 0; pass < npass; ++pass) {
         if (pass > 0) fock(c, vin);
@@ -1011,11 +1011,14 @@ int pd(Ctx& c, int k, double loose, const double* vin) {
         chk_launch(c);
         finish(c, ab, 3, 1 | (1 << 2) | (2 << 4), C_AOV);
         if (!read_ctl(c)) return 0;
-        const double tol = fmax(c.o.jacobi_tol * fmax(c.h[C_DMAX], 1.0), loose);
+        const double tol = fmax(c.o.jacobi_tol * fmax(c.h[C_DMAX], 1.0),
+                                loose > 0.0 ? loose : PD_FINAL_REL * c.o.x_atol_max);
         const double aov = c.h[C_AOV], kmax = c.h[C_KMAX], dmin = c.h[C_DMIN];
         if (aov <= tol) { ret = 1; break; }
         const int fail = pass == 0 ? 0 : -1;
         if (!(dmin > 0.0) || !(kmax <= c.o.pd_kmax)) { ret = fail; break; }
+        // an exact call's last pass only measures
+        if (loose == 0.0 && pass == npass - 1) { ret = fail; break; }
         double* X = c.T;
         double* Z = c.T + (size_t)k * k;
         gemm_sym<false, true>(c, k, m, Kb, m, Kb, m, X, k);                                  // K K^T
@@ -1063,7 +1066,7 @@ int pd(Ctx& c, int k, double loose, const double* vin) {
             double* t = cur; cur = nxt; nxt = t;
         }
         axpby(c, kn, 1.0, cur, 0.0, nullptr, c.Ct);
-        if (kmax * aov <= tol) { ret = 1; break; }
+        if (loose > 0.0 && kmax * aov <= tol) { ret = 1; break; }
     }
     c.t_pd += now_s() - t0;
     return ret;

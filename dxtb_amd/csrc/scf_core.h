@@ -38,16 +38,26 @@ struct Shared {
 // device wall clock (constant 100 MHz) in microseconds
 __device__ inline double now_us() { return (double)wall_clock64() * 0.01; }
 
-// Passes of the perturbative occupied-virtual rotation: an inexact SCF step
-// stops after two; the final (exact) step runs up to PD_EXACT_PASSES before
-// it falls back to a full eigensolve.  The passes converge linearly once the
-// occupied and virtual blocks are far from diagonal (the amplitudes use
-// their diagonals only): on 2400 molecules of the serving stream at the
-// default convergence threshold the final step needs 2 passes for 73 %, 3
-// for 18 %, 4-6 for 5 % of them, while a full eigensolve of a 500-orbital
-// molecule costs as much as ~15 passes.
+// Passes of the perturbative occupied-virtual rotation.  The amplitudes
+// divide by the diagonals of the occupied and virtual blocks only, and those
+// blocks drift away from diagonal over an SCF run, so the passes converge
+// linearly: the largest coupling falls by a factor 0.2 per pass (median;
+// 0.35-0.45 at the 90th percentile, 2400 molecules of the serving stream),
+// not quadratically.  An inexact SCF step stops after two passes, or once
+// the second-order estimate max|K| max|A_ov| is below its tolerance.  The
+// final step stops only on a measured coupling below
+// max(jacobi_tol max(|e|, 1), PD_FINAL_REL x_atol_max): a coupling d turns
+// the orbitals by ~d / gap (gaps >= 60 kT = 0.057 Eh here), so the charges
+// and forces move by < 40 d, at most what the SCF's own convergence
+// threshold on the potential leaves.  At x_atol_max = 1e-10, 79 % of the
+// 2400 molecules meet 1e-11 without a rotation and all within 6
+// measurements; 1e-2 x_atol_max needed up to 9 and more, and made that
+// launch 17 % longer.  After PD_EXACT_PASSES measurements the step falls
+// back to a full eigensolve, which costs as much as ~15 passes on a
+// 500-orbital molecule.
 constexpr int PD_INEXACT_PASSES = 2;
-constexpr int PD_EXACT_PASSES = 6;
+constexpr int PD_EXACT_PASSES = 7;
+constexpr double PD_FINAL_REL = 1e-1;
 
 // ---------------------------------------------------------------------------
 // Diagonal blocks of the blocked Cholesky orthogonaliser (one wave, 32 x 32

@@ -383,15 +383,14 @@ __device__ inline void fock_build(const Mol& M, const glbf* vin) {
 // with X = K K^T (k x k), Z = K C_vir (k x n), phi(x) = (cos sqrt x - 1)/x;
 // the functions of X are power series evaluated by Horner's rule, truncated
 // once the next term is below 1e-17, or 1e-13 in inexact steps (||X||_F
-// bounds the spectral norm).  The
-// coupling left is second order, about max|K| max|A_ov|; a second pass runs
-// when that estimate exceeds the tolerance.
+// bounds the spectral norm).  Another pass measures the coupling left and
+// rotates again (pass limits and exits: PD_INEXACT_PASSES, PD_EXACT_PASSES).
 // Returns 1 when done (rotated, or already decoupled to the tolerance), 0 to
 // ask for the Jacobi eigensolve with Ct and F untouched, -1 to ask for it
 // after F was used as scratch (the caller rebuilds F).  Jacobi is asked for
 // when an amplitude exceeds pd_kmax, a denominator is not positive, k > n/2,
-// a series would need more than 9 terms, or (exact calls, loose = 0)
-// PD_EXACT_PASSES passes leave a coupling above the tolerance.  On return 1, sh.ev holds the
+// a series would need more than 9 terms, or (exact calls, loose = 0) the
+// last measured coupling is above the tolerance.  On return 1, sh.ev holds the
 // diagonal of C F C^T in the rows' order (occupied rows: of the last pass).
 // Scratch: T holds C F, then X and Z; P the couplings/amplitudes; F the two
 // Horner accumulators (k x n each, k <= n/2).
@@ -468,7 +467,7 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         double dmax = 0.0;
         for (int i = threadIdx.x; i < n; i += NT) dmax = fmax(dmax, fabs(sh.ev[i]));
         dmax = wg_max(dmax, sh.red);
-        const double tol = fmax(o.jacobi_tol * fmax(dmax, 1.0), loose);
+        const double tol = fmax(o.jacobi_tol * fmax(dmax, 1.0), loose > 0.0 ? loose : PD_FINAL_REL * o.x_atol_max);
         double aov = 0.0, kmax = 0.0, dmin = INFINITY;
         for (int i0 = w * ROWS_PER_WAVE; i0 < k; i0 += NWAVE * ROWS_PER_WAVE) {
             int ii[ROWS_PER_WAVE];
@@ -502,6 +501,9 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
         const int fail = pass == 0 ? 0 : -1;
         PD_MARK(sh, 2, tm);
         if (!(dmin > 0.0) || !(kmax <= o.pd_kmax)) return fail;
+        // an exact call's last pass only measures: a rotation after it
+        // would go unchecked
+        if (loose == 0.0 && pass == npass - 1) return fail;
         glbf* X = M.T;
         glbf* Z = M.T + (size_t)k * k;
         gemm_sym<false, true>(k, m, 1.0, Kb, m, Kb, m, X, k, sh.big);
@@ -566,11 +568,12 @@ __device__ __noinline__ int pd_step(const Mol& M, const Shared& sh, const xtb_sc
 #ifdef XTB_PD_TIMERS
         sh.ptacc[9] += (double)J;
 #endif
-        if (kmax * aov <= tol) return 1;  // second-order remainder below the tolerance
+        // inexact steps: the second-order estimate of what is left
+        if (loose > 0.0 && kmax * aov <= tol) return 1;
     }
-    // the passes spent: good enough for an inexact step; an exact one
-    // (loose = 0) asks for Jacobi on the rotated orbitals instead
-    return loose > 0.0 ? 1 : -1;
+    // the passes spent: good enough for an inexact step (an exact one has
+    // returned from its last, measuring pass)
+    return 1;
 }
 
 // ---------------------------------------------------------------------------
