@@ -17,7 +17,10 @@ Requests (one molecule each, arbitrary size) arrive over time.  The server
   streams on one queue run their work one after the other;
 * keeps each molecule's SCF state (potential, Anderson history, orbitals,
   density) resident in that launch's device buffers until its results are
-  copied out, then frees the buffers and backfills with waiting requests.
+  copied out, then frees the buffers and backfills with waiting requests;
+* serves a request above the largest size class (1024 orbitals, the
+  per-molecule kernel's limit) on the whole-GPU path, one at a time, from a
+  worker thread on one of the streams, while the loop goes on.
 
 Every request's result is computed by exactly the same kernels whatever its
 neighbours are; ``tests/test_gpu_serving.py`` checks alone == in the mix.
@@ -27,6 +30,7 @@ from __future__ import annotations
 
 import functools
 import os
+import threading
 import time
 from collections import deque
 from dataclasses import dataclass, field
@@ -89,6 +93,7 @@ class ServerStats:
     host_admit_s: float = 0.0  # host time spent building and launching micro-batches
     host_retire_s: float = 0.0  # host time spent copying results out
     spans: list = field(default_factory=list)  # (start_us, duration_us) per molecule, when timing
+    large_runs: int = 0  # requests above the largest size class, served on the whole GPU
     launch_log: list = field(default_factory=list)  # per launch, when timing (see _retire)
 
 
@@ -200,6 +205,10 @@ class RaggedServer:
         self.results: dict[int, Request] = {}
         self.timing = timing
         self.kernel_log: list[tuple[float, float]] = []  # (ms, algorithmic flops) per SCF launch
+        # requests above the largest size class wait here for the whole-GPU
+        # lane, which runs one at a time: (request, thread, result box, stream)
+        self.large_queue: deque = deque()
+        self.large_job = None
 
     # ------------------------------------------------------------------ intake
     def submit(self, req: Request) -> None:
@@ -207,7 +216,9 @@ class RaggedServer:
             req.nao = orbital_count(req.numbers)
         k = int(np.searchsorted(self.edges, req.nao))
         if k >= len(self.edges):
-            raise ValueError(f"molecule with {req.nao} orbitals exceeds the largest size class")
+            # above the per-molecule kernel's classes: the whole-GPU lane
+            self.large_queue.append(req)
+            return
         self.queues[k].append(req)
 
     def submit_front(self, req: Request) -> None:
@@ -215,7 +226,57 @@ class RaggedServer:
         self.queues[int(np.searchsorted(self.edges, req.nao))].appendleft(req)
 
     def pending(self) -> int:
-        return sum(len(q) for q in self.queues)
+        return sum(len(q) for q in self.queues) + len(self.large_queue)
+
+    def busy(self) -> bool:
+        """Anything waiting, in flight or running in the whole-GPU lane."""
+        return bool(self.pending() or self.inflight or self.large_job is not None)
+
+    # ---------------------------------------------------------- whole-GPU lane
+    def _large_lane(self) -> list[Request]:
+        """Requests above the largest size class run one at a time through
+        the whole-GPU path (xtb_scf_large: grid kernels, an SCF loop on the
+        host that reads a few values back per step) on a stream taken from
+        the free ones, driven from a worker thread so that the loop keeps
+        admitting and retiring micro-batches meanwhile (the library call
+        releases the interpreter lock).  Returns the request it finished."""
+        done = []
+        job = self.large_job
+        if job is not None and not job[1].is_alive():
+            req, th, box, stream = job
+            th.join()
+            self.large_job = None
+            self.free_streams.append(stream)
+            if "error" in box:
+                raise box["error"]
+            rec = box["rec"]
+            e = float(rec[_lib.OUT_ESCF] + rec[_lib.OUT_FENERGY] + rec[_lib.OUT_EREP])
+            g = box["grad"]
+            self._finish(req, e, int(rec[_lib.OUT_STATUS]), int(rec[_lib.OUT_ITER]),
+                         None if g is None else -g, time.perf_counter())
+            self.stats.completed += 1
+            self.stats.large_runs += 1
+            done.append(req)
+        if self.large_job is None and self.large_queue and self.free_streams:
+            req = self.large_queue.popleft()
+            stream = self.free_streams.popleft()
+            box: dict = {}
+            th = threading.Thread(target=self._run_large, args=(req, stream, box), daemon=True)
+            self.large_job = (req, th, box, stream)
+            th.start()
+        return done
+
+    def _run_large(self, req: Request, stream, box: dict) -> None:
+        try:
+            with torch.cuda.stream(stream):
+                lay = BatchLayout([req.numbers], self.tables, padded=False, chrg=np.zeros(1))
+                pipe = Pipeline(lay, self.dtables, self.device, large=True)
+                pos = torch.as_tensor(np.ascontiguousarray(req.positions), dtype=torch.float64).to(self.device)
+                pipe.run(pos, self.opts, forces=self.forces, stream=stream)
+                box["rec"] = pipe.records()[0].cpu().numpy()
+                box["grad"] = pipe.gradient().cpu().numpy() if self.forces else None
+        except BaseException as exc:  # re-raised by the loop's thread
+            box["error"] = exc
 
     def _inflight_bytes(self) -> int:
         return sum(f.bytes for f in self.inflight)
@@ -267,7 +328,7 @@ class RaggedServer:
     def _admit(self, now: float, eager: bool = False) -> bool:
         if not self.free_streams:
             return False
-        if self.merge_above is not None and self.pending() > self.merge_above:
+        if self.merge_above is not None and sum(len(q) for q in self.queues) > self.merge_above:
             k = -1
             take = self._take_merged()
         else:
@@ -331,18 +392,8 @@ class RaggedServer:
         a0 = pipe.L.atom0
         now = time.perf_counter()
         for b, r in enumerate(f.reqs):
-            r.energy = float(e[b])
-            r.status = int(rec[b, 6])
-            r.iterations = int(rec[b, 5])
-            if fo is not None:
-                r.forces = fo[a0[b]:a0[b] + len(r.numbers)]
-            if r.failed:
-                r.energy = float("nan")
-                if r.forces is not None:
-                    r.forces = np.full_like(r.forces, np.nan)
-            r.done = now
-            self.results[r.rid] = r
-            self.stats.latencies.append(now - r.arrival)
+            self._finish(r, float(e[b]), int(rec[b, _lib.OUT_STATUS]), int(rec[b, _lib.OUT_ITER]),
+                         None if fo is None else fo[a0[b]:a0[b] + len(r.numbers)], now)
         self.stats.completed += len(f.reqs)
         self.stats.scf_busy_s += float(rec[:, _lib.OUT_T_TOTAL].sum()) * 1e-6
         if f.klass >= 0:
@@ -358,8 +409,21 @@ class RaggedServer:
         self.free_streams.append(f.stream)
         return f.reqs
 
+    def _finish(self, r: Request, energy: float, status: int, iterations: int, forces, now: float) -> None:
+        r.energy = energy
+        r.status = status
+        r.iterations = iterations
+        r.forces = forces
+        if r.failed:
+            r.energy = float("nan")
+            if r.forces is not None:
+                r.forces = np.full_like(r.forces, np.nan)
+        r.done = now
+        self.results[r.rid] = r
+        self.stats.latencies.append(now - r.arrival)
+
     def poll(self, block: bool = False) -> list[Request]:
-        out = []
+        out = self._large_lane()
         keep = []
         for f in self.inflight:
             if f.event.query():
@@ -397,6 +461,11 @@ class RaggedServer:
             self.submit(r)
         self.step(now, eager=True)
 
+    def _idle_wait(self) -> None:
+        """Only the whole-GPU lane is running: wait a little for it."""
+        if self.large_job is not None and not self.inflight:
+            time.sleep(2e-4)
+
     def serve_until(self, reqs: list[Request]) -> None:
         """Run the loop (admitting anything waiting, in arrival order) until
         every request of `reqs` has its result."""
@@ -405,12 +474,14 @@ class RaggedServer:
             self.step()
             if not self.free_streams or not self.pending():
                 self.poll(block=True)
+                self._idle_wait()
 
     def drain(self) -> None:
-        while self.pending() or self.inflight:
+        while self.busy():
             self.step()
             if self.inflight and not self.free_streams:
                 self.poll(block=True)
+            self._idle_wait()
 
     # ------------------------------------------------------------------ driver
     def run_stream(self, reqs: list[Request], realtime: bool = True) -> dict:
@@ -421,7 +492,7 @@ class RaggedServer:
         for r in pend:
             r.arrival = t0 + (r.arrival if realtime else 0.0)
         n = len(pend)
-        while pend or self.pending() or self.inflight:
+        while pend or self.busy():
             now = time.perf_counter()
             while pend and pend[0].arrival <= now:
                 self.submit(pend.popleft())
@@ -430,6 +501,8 @@ class RaggedServer:
                 self.poll(block=True)
             elif pend and not self.pending() and not self.inflight:
                 time.sleep(max(0.0, min(pend[0].arrival - time.perf_counter(), 0.001)))
+            elif not pend:
+                self._idle_wait()
         t1 = time.perf_counter()
         lat = np.array([self.results[r.rid].done - self.results[r.rid].arrival for r in reqs])
         return {

@@ -173,3 +173,29 @@ def test_result_cache_checks_positions_by_value():
             assert abs(e - fresh.get_energy(pos.clone()).item()) < 1e-12
             assert abs(calc.get_energy(pos).item() - e) < 1e-12
             del pos
+
+
+def test_request_above_the_largest_class_takes_the_whole_gpu_lane():
+    """A request above the largest size class (1024 orbitals) is served by
+    the whole-GPU path from a worker thread, while the loop keeps serving
+    the others: its result equals that path's alone, theirs equal their
+    own alone."""
+    from dxtb_amd.large import LargeSystem
+
+    rng = np.random.default_rng(21)
+    big = W.water_cluster(rng, 130)  # 1040 orbitals
+    stream = W.config4(seed=13, count=16)
+    reqs = [Request(0, big[0], big[1], arrival=0.0)]
+    reqs += [Request(i + 1, z, p, arrival=0.0) for i, (_, z, p) in enumerate(stream)]
+    print("case: serving with a 1040-orbital request", flush=True)
+    srv = RaggedServer("cuda:0", opts=ScfOptions(**OPTS), streams=4)
+    srv.run_stream(reqs, realtime=False)
+    assert srv.stats.large_runs == 1 and srv.stats.completed == len(reqs)
+    ref = LargeSystem(big[0], device="cuda:0", opts=ScfOptions(**OPTS)).run(big[1], forces=True)
+    got = srv.results[0]
+    assert got.status == 0 and ref["status"] == 0
+    assert abs(got.energy - ref["energy"]) < 1e-10
+    assert np.abs(got.forces + ref["gradient"].cpu().numpy()).max() < 1e-10
+    for r in reqs[1:7]:
+        alone = run_native([(r.numbers, r.positions)], forces=True, **OPTS)
+        assert abs(srv.results[r.rid].energy - alone.energies().item()) < 1e-11

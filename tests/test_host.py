@@ -219,6 +219,7 @@ def test_serving_merged_launch_takes_the_oldest_requests():
     srv = object.__new__(RaggedServer)
     srv.edges = DEFAULT_EDGES
     srv.queues = [deque() for _ in DEFAULT_EDGES]
+    srv.large_queue = deque()
     srv.merge_cap = 5
     srv.merge_largest = False
     z = np.array([1, 1])
@@ -229,6 +230,10 @@ def test_serving_merged_launch_takes_the_oldest_requests():
     for r in first + later:
         srv.submit(r)
     assert srv.pending() == 10
+    # above the largest class: the whole-GPU lane's queue, never a merged launch
+    srv.submit(mk(99, DEFAULT_EDGES[-1] + 1, 0.0))
+    assert srv.pending() == 11 and [r.rid for r in srv.large_queue] == [99]
+    srv.large_queue.clear()
     take = srv._take_merged()
     # class 5 (700), class 2 (150, 140), then class 0 in arrival order
     assert [r.rid for r in take] == [5, 3, 4, 0, 1]
