@@ -882,6 +882,75 @@ bool ortho(Ctx& c) {
     return it < 60;
 }
 
+// Warm start (as in k_scf): the orbitals a previous call left in the state
+// (rows in the orbital buffer) become S-orthonormal for the new geometry,
+// C <- (C S C^T)^-1/2 C by coupled Newton-Schulz (C S C^T is close to I
+// after a small displacement, so a few steps; the symmetric form keeps each
+// orbital as close to its old self as possible), and are sorted by their
+// energies in the Fock matrix of the kept potential `pot`, so that the first
+// step can decouple occupied from virtual orbitals without an eigensolve.
+// Returns false, with the orbitals untouched, when C S C^T is too far from
+// I (then the caller starts cold).
+bool warm_orthonormalise(Ctx& c, const double* pot) {
+    const int n = c.n;
+    const int rb = (n + NWAVE - 1) / NWAVE;
+    // G = C S C^T, the iteration's Y
+    gemm_t<false, false, false>(c, n, n, n, 1.0, c.Ct, n, c.S, n, 0.0, c.T, n, false);
+    gemm_sym<false, true>(c, n, n, c.T, n, c.Ct, n, c.F, n);
+    double* Y = c.F;
+    double* Z = c.P;
+    double* W = c.E;
+    double* T = c.T;
+    hipLaunchKernelGGL(k_identity, dim3((n + 255) / 256, n), dim3(256), 0, c.s, n, Z);
+    chk_launch(c);
+    int it = 0, extra = -1;
+    for (; it < 30; ++it) {
+        gemm_t<false, false, false>(c, n, n, n, -0.5, Z, n, Y, n, 0.0, T, n, false);
+        hipLaunchKernelGGL(k_add_diag, dim3((n + 255) / 256), dim3(256), 0, c.s, n, 1.5, T);
+        chk_launch(c);
+        hipLaunchKernelGGL(k_dev_identity, dim3(rb), dim3(NT), 0, c.s, n, T, c.part);
+        chk_launch(c);
+        finish(c, rb, 1, 1, C_ERR);
+        if (!read_ctl(c)) return false;
+        const double err = c.h[C_ERR];
+        // |I - T| = |I - G|/2 at the start: beyond 1/4 the kept orbitals
+        // are too far off to be worth keeping
+        if (it == 0 && !(err < 0.25)) return false;
+        if (extra == 0 || err < 1e-15) break;
+        if (extra > 0) --extra;
+        else if (err < 1e-9) extra = 1;
+        gemm_t<false, false, false>(c, n, n, n, 1.0, Y, n, T, n, 0.0, W, n, false);  // Y T
+        gemm_t<false, false, false>(c, n, n, n, 1.0, T, n, Z, n, 0.0, Y, n, false);  // T Z
+        double* oldZ = Z;
+        Z = Y;
+        Y = W;
+        W = oldZ;
+    }
+    if (it >= 30) return false;
+    // C <- Z C (Z = G^-1/2), through T
+    gemm_t<false, false, false>(c, n, n, n, 1.0, Z, n, c.Ct, n, 0.0, T, n, false);
+    chk(c, hipMemcpyAsync(c.Ct, T, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c.s));
+    // energy estimates e_i = (C F C^T)_ii, rows sorted by them; the sort's
+    // frontier gaps steer the first step's occupied-virtual decoupling
+    fock(c, pot);
+    gemm_t<false, false, false>(c, n, n, n, 1.0, c.Ct, n, c.F, n, 0.0, T, n, false);
+    hipLaunchKernelGGL(k_rowdot, dim3(rb), dim3(NT), 0, c.s, n, n, T, c.Ct, c.ev);
+    chk_launch(c);
+    const size_t lds = (256 + (size_t)c.npow) * 8 + (size_t)c.npow * 4;
+    const xtb_scf_opts& o = c.o;
+    hipLaunchKernelGGL(k_sort_fermi, dim3(1), dim3(NT), lds, c.s, n, c.nao, c.npow, c.ev, c.skey, c.sidx, c.ftot,
+                       c.st->occ, c.nel_a, c.nel_b, c.ia, c.ib, o.kt, o.fermi_thresh, o.fermi_maxiter, c.ctl);
+    chk_launch(c);
+    hipLaunchKernelGGL(k_gather, dim3(n), dim3(256), 0, c.s, n, c.Ct, c.sidx, c.F);
+    chk_launch(c);
+    chk(c, hipMemcpyAsync(c.Ct, c.F, sizeof(double) * n * n, hipMemcpyDeviceToDevice, c.s));
+    if (!read_ctl(c)) return false;
+    c.gap_a = c.h[C_GAPA];
+    c.gap_b = c.h[C_GAPB];
+    c.refresh = 1;  // the virtual orbitals' estimates are fresh
+    return true;
+}
+
 // Cholesky orthogonaliser (as chol_setup in scf.hip): S = L L^T by
 // right-looking 32-wide panels (a wave factors each diagonal block, the panel
 // solve and the trailing update are grid GEMMs), then X = L^-1 by the
@@ -1252,7 +1321,11 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
         attr = true;
     }
     double nel[2] = {0.0, 0.0};
+    double prev_status = 1.0;  // a warm start needs a previous call that succeeded
     chk(c, hipMemcpyAsync(nel, batch->mol_nel, sizeof(nel), hipMemcpyDeviceToHost, c.s));
+    if (opts->warm_start)
+        chk(c, hipMemcpyAsync(&prev_status, state->out + XTB_OUT_STATUS, sizeof(double), hipMemcpyDeviceToHost,
+                              c.s));
     chk(c, hipStreamSynchronize(c.s));
     if (c.err != hipSuccess) return XTB_EHIP;
     c.nel_a = nel[0];
@@ -1267,7 +1340,11 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
         rec.v[slot] = v;
         rec.mask |= 1u << slot;
     };
-    const bool ok_ortho = opts->ortho == 1 ? ortho_chol(c) : ortho(c);
+    // warm start from the orbitals and potential a previous call on this
+    // state left (a failed re-orthonormalisation starts cold)
+    bool warm = opts->warm_start != 0 && prev_status == 0.0;
+    if (warm) warm = warm_orthonormalise(c, state->pot);
+    const bool ok_ortho = warm || (opts->ortho == 1 ? ortho_chol(c) : ortho(c));
     const double t_ortho = now_s() - tstart;
     if (c.err != hipSuccess) return XTB_EHIP;
     if (!ok_ortho) {
@@ -1281,15 +1358,18 @@ extern "C" int32_t xtb_scf_large(const xtb_batch* batch, const xtb_params* par, 
         return c.err == hipSuccess ? XTB_OK : XTB_EHIP;
     }
     const int n = c.n;
-    chk(c, hipMemsetAsync(state->pot, 0, sizeof(double) * n, c.s));
+    if (!warm) chk(c, hipMemsetAsync(state->pot, 0, sizeof(double) * n, c.s));
     const xtb_scf_opts& o = c.o;
-    Mixer mx{0, 0, o.generations + 1, o.soft_start != 0};
+    Mixer mx{0, 0, o.generations + 1, !warm && o.soft_start != 0};
     Broyden bm{0, 0, o.generations, 0.0};
     const bool use_broyden = o.mixer == 1;
     int iters = 0, sweeps = 0;
     double resid = 1e-2;
-    const double loose0 = o.eig_inexact > 0.0 ? o.first_tol : 0.0;
-    const int sw_first = step(c, state->pot, state->vnew, status, loose0, false, true, false, nullptr, false);
+    // first step: a cold start diagonalises to first_tol; a warm start
+    // decouples to 1e-2 of that
+    const double loose0 = o.eig_inexact > 0.0 ? (warm ? 1e-2 * o.first_tol : o.first_tol) : 0.0;
+    const int sw_first = step(c, state->pot, state->vnew, status, loose0, warm && o.eig_inexact > 0.0, true, false,
+                              nullptr, false);
     sweeps += sw_first;
     ++iters;
     if (o.mix_guess || use_broyden) {

@@ -460,8 +460,6 @@ class Pipeline:
                 nw = ct.c_int64(0)
                 _lib.check(lib.xtb_scf_large_workspace(b, ct.byref(nw)), "xtb_scf_large_workspace")
                 self.work = torch.empty(int(nw.value), dtype=torch.float64, device=self.device)
-            o.keep_state = 0
-            o.warm_start = 0
             _lib.check(lib.xtb_scf_large(b, p, self.S.data_ptr(), self.H0.data_ptr(),
                                          self.gamma.data_ptr(), ct.byref(o), ct.byref(self.state),
                                          self.work.data_ptr(), self.work.numel(), s),
@@ -479,7 +477,7 @@ class Pipeline:
         if timing:
             ev1.record(st)
             self.scf_events = (ev0, ev1)
-        self._kept = bool(opts.warm_start) and not self.large
+        self._kept = bool(opts.warm_start)
         self._have_w = bool(forces)
         if forces and not fused:
             self.run_forces(stream)

@@ -313,8 +313,11 @@ int32_t xtb_scf_large_workspace(const xtb_batch* batch, int64_t* doubles);
  * SCF loop on the host (a few control values read back per step, so the
  * call returns after the SCF has finished on `stream`).  Same state buffers
  * (F, C, T, P: one n x n each, stored at offset 0) and result record as
- * xtb_scf, so xtb_forces and xtb_param_grad follow unchanged; keep_state and
- * warm_start are ignored.  `work` holds xtb_scf_large_workspace doubles.
+ * xtb_scf, so xtb_forces and xtb_param_grad follow unchanged.  The orbitals
+ * always stay in C (keep_state has no further effect); with warm_start and
+ * a previous call on this state that succeeded (its record's status 0), the
+ * SCF starts from them, re-orthonormalised against the new S, and from the
+ * kept potential.  `work` holds xtb_scf_large_workspace doubles.
  * Replaces the same upstream call sites as xtb_scf
  * (src/dxtb/_src/scf/iterator.py:51-144, scf/unrolling/default.py:71-401,
  * with scf/unrolling/base.py:142-175 for the eigensolve). */

@@ -204,3 +204,33 @@ def test_calculator_routes_a_single_molecule_by_size():
         assert (f + ref.gradient()[a0:a0 + na]).abs().max().item() < 1e-8
         assert (q - ref.orbital_charges(k)).abs().max().item() < 1e-8
     assert int(ref.L.n[0]) > LARGE_N >= int(ref.L.n[1])
+
+
+def test_large_path_warm_start_along_a_trajectory():
+    """One molecule on the whole-GPU path along a 3-geometry trajectory with
+    warm_start: each geometry starts from the orbitals the previous call
+    left (re-orthonormalised against the new overlap) and its potential.
+    Results equal a cold start's to the parity tolerances, and the first
+    step needs no eigensolve."""
+    rng = np.random.default_rng(14)
+    z, p0 = W.water_cluster(rng, 40)
+    geo = [p0]
+    for _ in range(2):
+        geo.append(geo[-1] + 0.03 * rng.standard_normal(p0.shape))
+    warm = LargeSystem(z, device="cuda:0", opts=ScfOptions(**TIGHT, warm_start=True))
+    cold = LargeSystem(z, device="cuda:0", opts=ScfOptions(**TIGHT))
+    for g, p in enumerate(geo):
+        print(f"case: whole-GPU warm start, geometry {g}", flush=True)
+        rw, rc = warm.run(p, forces=True), cold.run(p, forces=True)
+        assert rw["status"] == 0 and rc["status"] == 0
+        assert abs(rw["energy"] - rc["energy"]) < 1e-9
+        assert np.abs(rw["gradient"].cpu().numpy() - rc["gradient"].cpu().numpy()).max() < 1e-8
+        assert np.abs(rw["charges_orbital"].cpu().numpy() - rc["charges_orbital"].cpu().numpy()).max() < 1e-8
+        sw_w, sw_c = rw["records"][_lib.OUT_SWEEPS_FIRST], rc["records"][_lib.OUT_SWEEPS_FIRST]
+        print(f"geometry {g}: iterations warm {rw['iterations']} cold {rc['iterations']}, "
+              f"first-step sweeps warm {sw_w} cold {sw_c}", flush=True)
+        if g > 0:
+            # the saving is the first step's eigensolve; with random 0.03
+            # bohr displacements the iteration counts stay about the same
+            assert sw_w == 0 < sw_c
+            assert rw["iterations"] <= rc["iterations"] + 3
