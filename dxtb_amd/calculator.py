@@ -49,10 +49,10 @@ _DISP_WARNED = False
 # a single molecule with more orbitals than this runs its SCF on the whole
 # GPU (xtb_scf_large); batches use the per-molecule kernel, except for their
 # molecules above its limit (WG_MAX_N orbitals), which take the whole GPU
-# one after the other.  Energy + forces of one molecule alone, one CU /
-# whole GPU: n = 100: 4.6 / 7.0 ms, 154: 12.9 / 9.8, 236: 15.0 / 12.0,
-# 350: 38 / 18, 460: 70 / 20 (chains; water clusters alike)
-LARGE_N = 192
+# one after the other.  Energy + forces of one molecule alone at a 1e-10
+# threshold, one CU / whole GPU: n = 100: 5.6 / 6.8 ms, 154: 15.0 / 9.0,
+# 236: 19.9 / 11.5, 350: 53 / 16, 460: 99 / 18 (chains; water clusters alike)
+LARGE_N = 128
 
 
 def _opts_to_scf(opts: dict) -> tuple[ScfOptions, dict]:

@@ -73,7 +73,9 @@ __device__ inline void tile_of(int idx, int tn, int upper, int& ti, int& tj) {
     tj = r + rem;
 }
 
-template <bool TA, bool TB, bool EPI>
+// square tiles of TE x TE (128, or 64 for products too small to give a
+// quarter of the CUs a 128-tile)
+template <bool TA, bool TB, bool EPI, int TE>
 __global__ void __launch_bounds__(NT) k_gemm(int M, int N, int K, double alpha, const double* A, int lda,
                                              const double* B, int ldb, double beta, double* C, int ldc,
                                              int upper, int ntiles, const double* E1, const double* E2,
@@ -82,16 +84,18 @@ __global__ void __launch_bounds__(NT) k_gemm(int M, int N, int K, double alpha, 
     const int per = gridDim.x / 8;  // the grid is a multiple of 8
     const int idx = (blockIdx.x % 8) * per + blockIdx.x / 8;
     if (idx >= ntiles) return;
-    const int tn = (N + GT - 1) / GT;
+    const int tn = (N + TE - 1) / TE;
     int ti, tj;
     tile_of(idx, tn, upper, ti, tj);
-    gemm_tile<TA, TB, 4, 2, GRID_KP, EPI>(ti * GT, tj * GT, M, N, K, alpha, to_glb(A), lda, to_glb(B), ldb,
-                                          beta, to_glb(C), ldc, to_lds(lds), to_glb(E1), to_glb(E2), s1, s2);
+    gemm_tile<TA, TB, TE / 32, TE / 64, GRID_KP, EPI>(ti * TE, tj * TE, M, N, K, alpha, to_glb(A), lda, to_glb(B),
+                                                      ldb, beta, to_glb(C), ldc, to_lds(lds), to_glb(E1),
+                                                      to_glb(E2), s1, s2);
 }
 
-// C[j][i] = C[i][j] for the 64 x 64 blocks below the diagonal whose 128-tile
-// was not formed by an `upper` GEMM; one workgroup per block, through LDS.
-__global__ void __launch_bounds__(256) k_mirror(double* C, int n, int ldc) {
+// C[j][i] = C[i][j] for the 64 x 64 blocks below the diagonal whose tile
+// (tile edge `per` x 64) was not formed by an `upper` GEMM; one workgroup
+// per block, through LDS.
+__global__ void __launch_bounds__(256) k_mirror(double* C, int n, int ldc, int per) {
     __shared__ double t[64 * 65];
     const int nb = (n + 63) / 64;
     // block index -> (bi, bj), bi > bj, lower triangle row-major
@@ -101,7 +105,7 @@ __global__ void __launch_bounds__(256) k_mirror(double* C, int n, int ldc) {
         ++bi;
     }
     const int bj = rem;
-    if (bi >= nb || bi / 2 <= bj / 2) return;  // formed by the GEMM
+    if (bi >= nb || bi / per <= bj / per) return;  // formed by the GEMM
     for (int e = threadIdx.x; e < 64 * 64; e += 256) {
         const int r = e / 64, c = e % 64;  // source row bj*64 + r, column bi*64 + c
         const int gr = bj * 64 + r, gc = bi * 64 + c;
@@ -182,19 +186,22 @@ __global__ void __launch_bounds__(NT) k_rowabs(int n, const double* A, double* p
 // bits 2s..2s+1 of `ops`: 0 sum, 1 max, 2 min.
 __global__ void __launch_bounds__(NT) k_finish(const double* part, int nblk, int nslot, int ops, double* ctl,
                                                int ctl0) {
-    __shared__ double red[NT];
+    __shared__ double red[NWAVE];
+    const int w = threadIdx.x / WAVE, lane = threadIdx.x % WAVE;
     for (int s = 0; s < nslot; ++s) {
         const int op = (ops >> (2 * s)) & 3;
+        auto comb = [op](double x, double y) { return op == 0 ? x + y : (op == 1 ? fmax(x, y) : fmin(x, y)); };
         double a = op == 0 ? 0.0 : (op == 1 ? -INFINITY : INFINITY);
-        for (int b = threadIdx.x; b < nblk; b += NT) {
-            const double v = part[(size_t)b * nslot + s];
-            a = op == 0 ? a + v : (op == 1 ? fmax(a, v) : fmin(a, v));
-        }
-        red[threadIdx.x] = a;
+        for (int b = threadIdx.x; b < nblk; b += NT) a = comb(a, part[(size_t)b * nslot + s]);
+        // xor butterfly (each pair combined in the same order in both
+        // lanes: every lane ends with the same bits), then the waves in
+        // index order
+        for (int o = 1; o < WAVE; o <<= 1) a = comb(a, __shfl_xor(a, o, WAVE));
+        if (lane == 0) red[w] = a;
         __syncthreads();
         if (threadIdx.x == 0) {
             double r = red[0];
-            for (int k = 1; k < NT; ++k) r = op == 0 ? r + red[k] : (op == 1 ? fmax(r, red[k]) : fmin(r, red[k]));
+            for (int k = 1; k < NWAVE; ++k) r = comb(r, red[k]);
             ctl[ctl0 + s] = r;
         }
         __syncthreads();
@@ -795,20 +802,31 @@ void gemm_t(Ctx& c, int M, int N, int K, double alpha, const double* A, int lda,
     if (M <= 0 || N <= 0) return;
     static bool attr = false;
     if (!attr) {
-        chk(c, hipFuncSetAttribute((const void*)k_gemm<TA, TB, EPI>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        chk(c, hipFuncSetAttribute((const void*)k_gemm<TA, TB, EPI, GT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                   GEMM_LDS_BYTES));
+        chk(c, hipFuncSetAttribute((const void*)k_gemm<TA, TB, EPI, 64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                    GEMM_LDS_BYTES));
         attr = true;
     }
-    const int tm = (M + GT - 1) / GT, tn = (N + GT - 1) / GT;
-    const int ntiles = upper ? tm * (tm + 1) / 2 : tm * tn;
+    auto count = [&](int te) {
+        const int tm = (M + te - 1) / te, tn = (N + te - 1) / te;
+        return upper ? tm * (tm + 1) / 2 : tm * tn;
+    };
+    // fewer 128-tiles than a quarter of the CUs: 64-tiles, four times as many
+    const int te = count(GT) < 64 ? 64 : GT;
+    const int ntiles = count(te);
     const int grid = (ntiles + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_gemm<TA, TB, EPI>), dim3(grid), dim3(NT), GEMM_LDS_BYTES, c.s, M, N, K, alpha, A, lda,
-                       B, ldb, beta, C, ldc, upper ? 1 : 0, ntiles, E1, E2, s1, s2);
+    if (te == GT)
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, GT>), dim3(grid), dim3(NT), GEMM_LDS_BYTES, c.s, M, N, K, alpha, A,
+                           lda, B, ldb, beta, C, ldc, upper ? 1 : 0, ntiles, E1, E2, s1, s2);
+    else
+        hipLaunchKernelGGL((k_gemm<TA, TB, EPI, 64>), dim3(grid), dim3(NT), GEMM_LDS_BYTES, c.s, M, N, K, alpha, A,
+                           lda, B, ldb, beta, C, ldc, upper ? 1 : 0, ntiles, E1, E2, s1, s2);
     chk_launch(c);
     if (upper) {
         const int nb = (M + 63) / 64;
         if (nb > 1) {
-            hipLaunchKernelGGL(k_mirror, dim3(nb * (nb - 1) / 2), dim3(256), 0, c.s, C, M, ldc);
+            hipLaunchKernelGGL(k_mirror, dim3(nb * (nb - 1) / 2), dim3(256), 0, c.s, C, M, ldc, te / 64);
             chk_launch(c);
         }
     }
