@@ -315,14 +315,9 @@ __global__ void __launch_bounds__(NT) k_dev_identity(int n, const double* T, dou
 // inverse to Dg (nb x nb, leading dimension nb); a non-positive pivot sets
 // ctl[C_ERR].  With factor = 0 it only inverts the (already factored) block.
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(WAVE) k_chol_diag(double* L_, int n, int j0, int factor, double* Dg_,
-                                                    double* ctl) {
+__device__ inline void chol_diag_block(ldsf* D, ldsf* Di, glbf* L, int n, int j0, int factor, glbf* Dg,
+                                       double* ctl, int lane) {
     constexpr int NB = 32;
-    __shared__ double dl[NB * NB], dil[NB * NB];
-    ldsf* D = to_lds(dl);
-    ldsf* Di = to_lds(dil);
-    glbf* L = to_glb(L_);
-    const int lane = threadIdx.x;
     const int nb = n - j0 < NB ? n - j0 : NB;
     for (int e = lane; e < NB * NB; e += WAVE) {
         const int r = e / NB, c = e % NB;
@@ -339,7 +334,24 @@ __global__ void __launch_bounds__(WAVE) k_chol_diag(double* L_, int n, int j0, i
         }
         wave_sync();
     }
-    chol_block_inverse(D, Di, to_glb(Dg_), nb, lane);
+    chol_block_inverse(D, Di, Dg, nb, lane);
+}
+
+__global__ void __launch_bounds__(WAVE) k_chol_diag(double* L_, int n, int j0, int factor, double* Dg_,
+                                                    double* ctl) {
+    __shared__ double dl[32 * 32], dil[32 * 32];
+    chol_diag_block(to_lds(dl), to_lds(dil), to_glb(L_), n, j0, factor, to_glb(Dg_), ctl, threadIdx.x);
+}
+
+// The inverses of all diagonal blocks of a finished factor L at once (they
+// are independent): block j (rows 32 j ..) to Dg + 32 j * 32, nb x nb with
+// leading dimension nb; one wave per block.  Dg holds 32 n doubles (n >= 32)
+// or n^2.
+__global__ void __launch_bounds__(WAVE) k_chol_inv_all(double* L_, int n, double* Dg_) {
+    __shared__ double dl[32 * 32], dil[32 * 32];
+    const int j0 = blockIdx.x * 32;
+    chol_diag_block(to_lds(dl), to_lds(dil), to_glb(L_), n, j0, 0, to_glb(Dg_ + (size_t)j0 * 32), nullptr,
+                    threadIdx.x);
 }
 
 // Y[r][c] = X[r][c] for r < rows, c < cols (leading dimensions ldx, ldy)
@@ -1002,13 +1014,14 @@ bool ortho_chol(Ctx& c) {
     if (!read_ctl(c) || c.h[C_ERR] != 0.0) return false;
     hipLaunchKernelGGL(k_identity, dim3((n + 255) / 256, n), dim3(256), 0, c.s, n, X);
     chk_launch(c);
+    hipLaunchKernelGGL(k_chol_inv_all, dim3((n + NB - 1) / NB), dim3(WAVE), 0, c.s, L, n, Dg);
+    chk_launch(c);
     for (int j0 = 0; j0 < n; j0 += NB) {
         const int nb = n - j0 < NB ? n - j0 : NB;
         const int w = j0 + nb;  // non-zero columns of block row j
-        hipLaunchKernelGGL(k_chol_diag, dim3(1), dim3(WAVE), 0, c.s, L, n, j0, 0, Dg, c.ctl);
-        chk_launch(c);
         double* Xj = X + (size_t)j0 * n;
-        gemm_t<false, false, false>(c, nb, w, nb, 1.0, Dg, nb, Xj, n, 0.0, R, n, false);  // D^-1 X_j
+        gemm_t<false, false, false>(c, nb, w, nb, 1.0, Dg + (size_t)j0 * NB, nb, Xj, n, 0.0, R, n,
+                                    false);  // D^-1 X_j
         hipLaunchKernelGGL(k_copy_block, dim3((w + 255) / 256, nb), dim3(256), 0, c.s, nb, w, R, n, Xj, n);
         chk_launch(c);
         const int mt = n - w;
