@@ -234,3 +234,11 @@ def test_large_path_warm_start_along_a_trajectory():
             # bohr displacements the iteration counts stay about the same
             assert sw_w == 0 < sw_c
             assert rw["iterations"] <= rc["iterations"] + 3
+    # a large jump (0.4 bohr per coordinate): whether the kept orbitals are
+    # kept or dropped for a cold start, the result is a cold start's
+    p = geo[-1] + 0.4 * rng.standard_normal(p0.shape)
+    print("case: whole-GPU warm start after a 0.4 bohr jump", flush=True)
+    rw, rc = warm.run(p, forces=True), cold.run(p, forces=True)
+    assert rw["status"] == 0 and rc["status"] == 0
+    assert abs(rw["energy"] - rc["energy"]) < 1e-9
+    assert np.abs(rw["gradient"].cpu().numpy() - rc["gradient"].cpu().numpy()).max() < 1e-8
